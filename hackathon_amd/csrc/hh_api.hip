@@ -1,0 +1,873 @@
+// C-ABI layer of the MI355X compact-Helmholtz library (include/hh_solver.h).
+//
+// Contexts own the stream, the per-length twiddle tables, the dense DST
+// matrices of non power-of-two sizes and a growable scratch buffer.  Plans
+// own the device copies of the O(N) stencil coefficient vectors and the
+// eigenvalues 2cos(k pi/(N+1)); they replace the reference's LU() factors.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/hh_solver.h"
+#include "hh_internal.h"
+
+using cd = std::complex<double>;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                               \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(HH_ERR_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),     \
+                        __FILE__, __LINE__);                                                       \
+    } while (0)
+
+struct hh_context {
+    int device = 0;
+    hipStream_t own = nullptr, stream = nullptr;
+    std::map<int, void *> tw64, tw32;   // twiddles by FFT length
+    std::map<int, void *> S64, S32;     // dense DST matrices by N
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    double *d_norm = nullptr;
+    size_t budget = 0, in_use = 0;
+    std::mutex mu;
+    // kernel timing (hh_context_enable_timing)
+    struct Rec { hipEvent_t a, b; int cat; double bytes; };
+    bool timing = false;
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    double acc_ms[5] = {0, 0, 0, 0, 0}, acc_bytes[5] = {0, 0, 0, 0, 0};
+    long acc_n[5] = {0, 0, 0, 0, 0};
+};
+
+// Records a pair of events around the launches in its scope when timing is on.
+struct TimeScope {
+    hh_context *c;
+    hh_context::Rec r;
+    bool on;
+    TimeScope(hh_context *ctx, int cat, double bytes) : c(ctx), on(ctx->timing) {
+        if (!on) return;
+        r.cat = cat;
+        r.bytes = bytes;
+        r.a = take();
+        r.b = take();
+        (void)hipEventRecord(r.a, c->stream);
+    }
+    hipEvent_t take() {
+        if (!c->pool.empty()) { hipEvent_t e = c->pool.back(); c->pool.pop_back(); return e; }
+        hipEvent_t e = nullptr;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    ~TimeScope() {
+        if (!on) return;
+        (void)hipEventRecord(r.b, c->stream);
+        c->recs.push_back(r);
+    }
+};
+
+struct hh_plan {
+    hh_context *ctx = nullptr;
+    int dim = 2, Nx = 0, Ny = 0, Nz = 1, prec = HH_PREC_F64;
+    void *coef64 = nullptr, *coef32 = nullptr;   // [q][6] or [q][12]
+    double *lamx = nullptr, *lamy = nullptr;
+    std::vector<cd> hcoef;                       // host copy of the band table
+    double tol = 1e-12;
+    int max_iter = 6;
+    int last_sweeps = 0;
+    double last_rel = 0;
+};
+
+// ------------------------------------------------------------------ helpers
+static int dev_alloc(hh_context *c, void **p, size_t bytes) {
+    if (c->budget && c->in_use + bytes > c->budget)
+        return fail(HH_ERR_NOMEM, "allocation of %zu bytes exceeds the memory budget (%zu of %zu in use)", bytes,
+                    c->in_use, c->budget);
+    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) return fail(HH_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    c->in_use += bytes;
+    return HH_OK;
+}
+
+static void dev_free(hh_context *c, void *p, size_t bytes) {
+    if (!p) return;
+    (void)hipFree(p);
+    c->in_use -= bytes < c->in_use ? bytes : c->in_use;
+}
+
+static int ensure_scratch(hh_context *c, size_t bytes) {
+    if (c->scratch_bytes >= bytes) return HH_OK;
+    if (c->scratch) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        dev_free(c, c->scratch, c->scratch_bytes);
+        c->scratch = nullptr;
+        c->scratch_bytes = 0;
+    }
+    int rc = dev_alloc(c, &c->scratch, bytes);
+    if (rc) return rc;
+    c->scratch_bytes = bytes;
+    return HH_OK;
+}
+
+static int set_device(hh_context *c) {
+    HIPCHK(hipSetDevice(c->device));
+    return HH_OK;
+}
+
+// e^{-2 pi i q/L}, q < L, with exact argument reduction to the first octant
+static void twiddles(int L, std::vector<cd> &out) {
+    out.resize(L);
+    for (int q = 0; q < L; q++) {
+        long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)q / (long double)L;
+        out[q] = cd((double)cosl(a), (double)sinl(a));
+    }
+}
+
+static int get_twiddles(hh_context *c, int N, int f32, void **out) {
+    int L = 2 * (N + 1);
+    auto &m = f32 ? c->tw32 : c->tw64;
+    auto it = m.find(L);
+    if (it != m.end()) { *out = it->second; return HH_OK; }
+    std::vector<cd> tw;
+    twiddles(L, tw);
+    void *d = nullptr;
+    size_t bytes = (size_t)L * (f32 ? 8 : 16);
+    int rc = dev_alloc(c, &d, bytes);
+    if (rc) return rc;
+    if (f32) {
+        std::vector<float> h(2 * L);
+        for (int q = 0; q < L; q++) { h[2 * q] = (float)tw[q].real(); h[2 * q + 1] = (float)tw[q].imag(); }
+        HIPCHK(hipMemcpy(d, h.data(), bytes, hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(hipMemcpy(d, tw.data(), bytes, hipMemcpyHostToDevice));
+    }
+    m[L] = d;
+    *out = d;
+    return HH_OK;
+}
+
+// S[n][k] = sqrt(2/(N+1)) sin(pi (n+1)(k+1)/(N+1)), exact integer reduction
+static int get_dst_matrix(hh_context *c, int N, int f32, void **out) {
+    auto &m = f32 ? c->S32 : c->S64;
+    auto it = m.find(N);
+    if (it != m.end()) { *out = it->second; return HH_OK; }
+    std::vector<double> S((size_t)N * N);
+    long M2 = 2L * (N + 1);
+    double s = std::sqrt(2.0 / (N + 1));
+    for (long n = 0; n < N; n++)
+        for (long k = 0; k < N; k++) {
+            long q = ((n + 1) * (k + 1)) % M2;
+            S[n * N + k] = s * (double)sinl(3.14159265358979323846264338327950288L * (long double)q / (long double)(N + 1));
+        }
+    void *d = nullptr;
+    size_t bytes = (size_t)N * N * (f32 ? 4 : 8);
+    int rc = dev_alloc(c, &d, bytes);
+    if (rc) return rc;
+    if (f32) {
+        std::vector<float> h(S.begin(), S.end());
+        HIPCHK(hipMemcpy(d, h.data(), bytes, hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(hipMemcpy(d, S.data(), bytes, hipMemcpyHostToDevice));
+    }
+    m[N] = d;
+    *out = d;
+    return HH_OK;
+}
+
+// One DST pass over lines; in-place when the FFT path applies.
+static int dst_pass(hh_context *c, const hh::Lines &ln, int ti, int tc, int to, const void *in, void *out) {
+    void *tw = nullptr, *S = nullptr;
+    int rc;
+    if (hh::fft_supported(ln.N)) rc = get_twiddles(c, ln.N, tc, &tw);
+    else rc = get_dst_matrix(c, ln.N, tc, &S);
+    if (rc) return rc;
+    const double elems = (double)ln.N * ln.ncols * ln.nplanes;
+    TimeScope ts(c, tw ? 0 : 1, elems * ((ti ? 8 : 16) + (to ? 8 : 16)));
+    hipError_t e = hh::launch_dst(ln, hh::DstTypes{ti, tc, to}, in, out, tw, S, c->stream);
+    if (e != hipSuccess) return fail(HH_ERR_DEVICE, "DST launch (N=%d) failed: %s", ln.N, hipGetErrorString(e));
+    return HH_OK;
+}
+
+static hh::Lines rows_lines(int N, long nrows) {
+    hh::Lines l;
+    l.N = N; l.ncols = nrows; l.nplanes = 1; l.plane_stride = (long)N * nrows; l.col_stride = N; l.elem_stride = 1;
+    return l;
+}
+
+static hh::Lines cols_lines(int Nx, int Ny, long nplanes) {
+    hh::Lines l;
+    l.N = Ny; l.ncols = Nx; l.nplanes = nplanes; l.plane_stride = (long)Nx * Ny; l.col_stride = 1; l.elem_stride = Nx;
+    return l;
+}
+
+static bool plan_all_fft(const hh_plan *p) {
+    return hh::fft_supported(p->Nx) && (p->dim == 2 || hh::fft_supported(p->Ny));
+}
+
+// ------------------------------------------------------------------ errors
+extern "C" const char *hh_last_error(void) { return g_err.c_str(); }
+
+extern "C" const char *hh_status_string(int s) {
+    switch (s) {
+    case HH_OK: return "ok";
+    case HH_ERR_ARG: return "invalid argument";
+    case HH_ERR_DEVICE: return "device error";
+    case HH_ERR_NOMEM: return "out of device memory";
+    case HH_ERR_UNSUPPORTED: return "unsupported shape";
+    case HH_ERR_SINGULAR: return "singular mode";
+    default: return "unknown status";
+    }
+}
+
+// ------------------------------------------------------------------ context
+extern "C" int hh_context_create(int device, hh_context **out) {
+    if (!out) return fail(HH_ERR_ARG, "hh_context_create: null output pointer");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return fail(HH_ERR_DEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= n) return fail(HH_ERR_ARG, "device %d out of range (%d devices)", device, n);
+    hh_context *c = new hh_context();
+    c->device = device;
+    HIPCHK(hipSetDevice(device));
+    e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete c; return fail(HH_ERR_DEVICE, "stream creation failed: %s", hipGetErrorString(e)); }
+    c->stream = c->own;
+    if (hipMalloc((void **)&c->d_norm, 2 * sizeof(double)) != hipSuccess) {
+        (void)hipStreamDestroy(c->own);
+        delete c;
+        return fail(HH_ERR_NOMEM, "norm buffer allocation failed");
+    }
+    *out = c;
+    return HH_OK;
+}
+
+extern "C" int hh_context_destroy(hh_context *c) {
+    if (!c) return HH_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &kv : c->tw64) (void)hipFree(kv.second);
+    for (auto &kv : c->tw32) (void)hipFree(kv.second);
+    for (auto &kv : c->S64) (void)hipFree(kv.second);
+    for (auto &kv : c->S32) (void)hipFree(kv.second);
+    if (c->scratch) (void)hipFree(c->scratch);
+    for (auto &r : c->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (auto e : c->pool) (void)hipEventDestroy(e);
+    (void)hipFree(c->d_norm);
+    (void)hipStreamDestroy(c->own);
+    delete c;
+    return HH_OK;
+}
+
+extern "C" int hh_context_set_stream(hh_context *c, void *stream) {
+    if (!c) return fail(HH_ERR_ARG, "null context");
+    c->stream = stream ? (hipStream_t)stream : c->own;
+    return HH_OK;
+}
+
+extern "C" void *hh_context_get_stream(hh_context *c) { return c ? (void *)c->stream : nullptr; }
+
+extern "C" int hh_context_synchronize(hh_context *c) {
+    if (!c) return fail(HH_ERR_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return HH_OK;
+}
+
+extern "C" int hh_context_set_memory_budget(hh_context *c, size_t bytes) {
+    if (!c) return fail(HH_ERR_ARG, "null context");
+    c->budget = bytes;
+    return HH_OK;
+}
+
+extern "C" size_t hh_context_bytes_in_use(const hh_context *c) { return c ? c->in_use : 0; }
+
+static int collect_timing(hh_context *c) {
+    if (c->recs.empty()) return HH_OK;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (auto &r : c->recs) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+        c->acc_ms[r.cat] += ms;
+        c->acc_bytes[r.cat] += r.bytes;
+        c->acc_n[r.cat] += 1;
+        c->pool.push_back(r.a);
+        c->pool.push_back(r.b);
+    }
+    c->recs.clear();
+    return HH_OK;
+}
+
+extern "C" int hh_context_enable_timing(hh_context *c, int on) {
+    if (!c) return fail(HH_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->timing = on != 0;
+    return HH_OK;
+}
+
+extern "C" int hh_context_read_timing(hh_context *c, double ms[5], long launches[5], double bytes[5]) {
+    if (!c) return fail(HH_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = collect_timing(c);
+    if (rc) return rc;
+    for (int k = 0; k < 5; k++) {
+        if (ms) ms[k] = c->acc_ms[k];
+        if (launches) launches[k] = c->acc_n[k];
+        if (bytes) bytes[k] = c->acc_bytes[k];
+    }
+    return HH_OK;
+}
+
+extern "C" int hh_context_reset_timing(hh_context *c) {
+    if (!c) return fail(HH_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = collect_timing(c);
+    for (int k = 0; k < 5; k++) { c->acc_ms[k] = 0; c->acc_bytes[k] = 0; c->acc_n[k] = 0; }
+    return rc;
+}
+
+// ------------------------------------------------------------------ coefficients
+static inline cd ld(const double *p, long i) { return p ? cd(p[2 * i], p[2 * i + 1]) : cd(0, 0); }
+static inline void st(double *p, long i, cd v) { p[2 * i] = v.real(); p[2 * i + 1] = v.imag(); }
+
+extern "C" int hh_coefficients2d(int order, int Nx, int Ny, double hx, double hy, const double *k2e,
+                                 const double *k2y, const double *k2yy, double *bp, double *ap, double *b, double *a,
+                                 double *bm, double *am) {
+    if (Nx < 1 || Ny < 1 || !k2e || !bp || !ap || !b || !a || !bm || !am)
+        return fail(HH_ERR_ARG, "hh_coefficients2d: bad arguments");
+    if (order != 2 && order != 4 && order != 6) return fail(HH_ERR_ARG, "order must be 2, 4 or 6 (got %d)", order);
+    const double hy2 = hy * hy, hy3 = hy2 * hy, hy4 = hy2 * hy2, Ryx = hy * hy / hx / hx;
+    for (int j = 0; j < Ny; j++) {
+        cd k2 = ld(k2e, j + 1), k2p = ld(k2e, j + 2), k2m = ld(k2e, j);
+        if (order == 6) {                                   // coefficients.c:44-61
+            cd ky2 = ld(k2y, j), kyy = ld(k2yy, j);
+            st(ap, j, 2.0 / 3.0 + hy2 * k2p / 90.0 + hy3 * ky2 * (hy2 * k2p / 6.0 + 2.0 / 3.0) / 20.0);
+            st(bp, j, 1.0 / 6.0 + hy2 * k2p / 90.0 + hy3 * ky2 / 120.0);
+            st(a, j, -10.0 / 3.0 + 41.0 * hy2 * k2 / 45.0 - hy4 * k2 * k2 / 20.0 + hy4 * kyy / 20.0);
+            st(b, j, 2.0 / 3.0 + hy2 * k2 / 90.0);
+            st(am, j, 2.0 / 3.0 + hy2 * k2m / 90.0 - hy3 * ky2 * (hy2 * k2m / 6.0 + 2.0 / 3.0) / 20.0);
+            st(bm, j, 1.0 / 6.0 + hy2 * k2m / 90.0 - hy3 * ky2 / 120.0);
+        } else if (order == 4) {                            // coefficients.c:25-42
+            st(bp, j, cd(Ryx / 12.0 + 1.0 / 12.0));
+            st(ap, j, 5.0 / 6.0 - Ryx / 6.0 + hy2 * k2p / 12.0);
+            st(b, j, Ryx * (5.0 / 6.0) - 1.0 / 6.0 + hy2 * k2 / 12.0);
+            st(a, j, -Ryx * (5.0 / 3.0) - 5.0 / 3.0 + 2.0 * hy2 * k2 / 3.0);
+            st(bm, j, cd(Ryx / 12.0 + 1.0 / 12.0));
+            st(am, j, 5.0 / 6.0 - Ryx / 6.0 + hy2 * k2m / 12.0);
+        } else {                                            // coefficients.c:9-23
+            st(bp, j, cd(0));
+            st(ap, j, cd(1));
+            st(b, j, cd(Ryx));
+            st(a, j, -2.0 * Ryx - 2.0 + k2 * hy2);
+            st(bm, j, cd(0));
+            st(am, j, cd(1));
+        }
+    }
+    return HH_OK;
+}
+
+// compact (1, diag, 1) solve of a complex vector (4th_order_partials.c:322-336)
+static void compact_line(std::vector<cd> &v, double diag) {
+    int n = (int)v.size();
+    std::vector<double> U(n);
+    U[0] = diag;
+    for (int j = 1; j < n; j++) U[j] = diag - 1.0 / U[j - 1];
+    for (int j = 1; j < n; j++) v[j] = v[j] - (1.0 / U[j - 1]) * v[j - 1];
+    v[n - 1] = v[n - 1] / U[n - 1];
+    for (int j = n - 2; j >= 0; j--) v[j] = (v[j] - v[j + 1]) / U[j];
+}
+
+extern "C" int hh_coefficients3d(int order, int Nx, int Ny, int Nz, double hx, double hy, double hz,
+                                 const double *k2e, const double *k2d, double *const coef[12]) {
+    if (Nx < 1 || Ny < 1 || Nz < 1 || !k2e || !coef) return fail(HH_ERR_ARG, "hh_coefficients3d: bad arguments");
+    for (int q = 0; q < 12; q++)
+        if (!coef[q]) return fail(HH_ERR_ARG, "hh_coefficients3d: null coefficient array %d", q);
+    if (order != 2 && order != 4 && order != 6) return fail(HH_ERR_ARG, "order must be 2, 4 or 6 (got %d)", order);
+    double *A = coef[0], *B = coef[1], *C = coef[2], *D = coef[3], *Ap = coef[4], *Bp = coef[5], *Cp = coef[6],
+           *Dp = coef[7], *Am = coef[8], *Bm = coef[9], *Cm = coef[10], *Dm = coef[11];
+    const double Rzx = hz * hz / hx / hx, Rzy = hz * hz / hy / hy, h2 = hz * hz;
+    std::vector<cd> k2(Nz + 2);
+    for (int l = 0; l < Nz + 2; l++) k2[l] = ld(k2e, l);
+    if (order == 6) {                                       // coefficients.c:67-103
+        std::vector<cd> kp(Nz), kpp(Nz);
+        for (int l = 0; l < Nz; l++) {
+            kp[l] = 3.0 * (k2[l + 2] - k2[l]) / hz;
+            kpp[l] = 12.0 * (k2[l] - 2.0 * k2[l + 1] + k2[l + 2]) / hz / hz;
+        }
+        kp[0] -= ld(k2d, 0); kp[Nz - 1] -= ld(k2d, 1);
+        kpp[0] -= ld(k2d, 2); kpp[Nz - 1] -= ld(k2d, 3);
+        compact_line(kp, 4.0);
+        compact_line(kpp, 10.0);
+        for (int i = 0; i < Nz; i++) {
+            st(A, i, 1.0 / 10.0 + h2 * k2[i + 1] / 90.0);
+            st(B, i, 7.0 / 15.0 - h2 * k2[i + 1] / 90.0);
+            st(C, i, 7.0 / 15.0 - h2 * k2[i + 1] / 90.0);
+            st(D, i, -64.0 / 15.0 + 14.0 * h2 * k2[i + 1] / 15.0 - h2 * h2 * k2[i + 1] * k2[i + 1] / 20.0 +
+                         h2 * h2 * kpp[i] / 20.0);
+            st(Ap, i, cd(1.0 / 30.0));
+            st(Bp, i, 1.0 / 10.0 + h2 * k2[i + 2] / 90.0 + (kp[i] * h2 * hz / 120.0));
+            st(Cp, i, 1.0 / 10.0 + h2 * k2[i + 2] / 90.0 + (kp[i] * h2 * hz / 120.0));
+            st(Dp, i, 7.0 / 15.0 - h2 * k2[i + 2] / 90.0 + (kp[i] * h2 * hz / 20.0) * (1.0 / 3.0 + k2[i + 2] * h2 / 6.0));
+            st(Am, i, cd(1.0 / 30.0));
+            st(Bm, i, 1.0 / 10.0 + h2 * k2[i] / 90.0 - (kp[i] * h2 * hz / 120.0));
+            st(Cm, i, 1.0 / 10.0 + h2 * k2[i] / 90.0 - (kp[i] * h2 * hz / 120.0));
+            st(Dm, i, 7.0 / 15.0 - h2 * k2[i] / 90.0 - (kp[i] * h2 * hz / 20.0) * (1.0 / 3.0 + k2[i] * h2 / 6.0));
+        }
+    } else if (order == 4) {                                // coefficients.c:35-65
+        for (int i = 0; i < Nz; i++) {
+            st(A, i, cd((1.0 / 12.0) * (Rzx + Rzy)));
+            st(B, i, (1.0 / 6.0) * (4.0 * Rzx - Rzy - 1.0 + 0.5 * h2 * k2[i + 1]));
+            st(C, i, (1.0 / 6.0) * (4.0 * Rzy - Rzx - 1.0 + 0.5 * h2 * k2[i + 1]));
+            st(D, i, -4.0 * (1.0 + Rzx + Rzy) / 3.0 + 0.5 * h2 * k2[i + 1]);
+            st(Ap, i, cd(0));
+            st(Bp, i, cd((1.0 / 12.0) * (1.0 + Rzx)));
+            st(Cp, i, cd((1.0 / 12.0) * (1.0 + Rzy)));
+            st(Dp, i, 1.0 - (1.0 / 6.0) * (2.0 + Rzx + Rzy) + (1.0 / 12.0) * h2 * k2[i + 2]);
+            st(Am, i, cd(0));
+            st(Bm, i, cd((1.0 / 12.0) * (1.0 + Rzx)));
+            st(Cm, i, cd((1.0 / 12.0) * (1.0 + Rzy)));
+            st(Dm, i, 1.0 - (1.0 / 6.0) * (2.0 + Rzx + Rzy) + (1.0 / 12.0) * h2 * k2[i]);
+        }
+    } else {                                                // coefficients.c:7-33
+        for (int i = 0; i < Nz; i++) {
+            st(A, i, cd(0)); st(B, i, cd(Rzx)); st(C, i, cd(Rzy));
+            st(D, i, -2.0 * (Rzx + Rzy + 1.0) + h2 * k2[i + 1]);
+            st(Ap, i, cd(0)); st(Bp, i, cd(0)); st(Cp, i, cd(0)); st(Dp, i, cd(1));
+            st(Am, i, cd(0)); st(Bm, i, cd(0)); st(Cm, i, cd(0)); st(Dm, i, cd(1));
+        }
+    }
+    return HH_OK;
+}
+
+// ------------------------------------------------------------------ plans
+static int upload_plan(hh_plan *p, int ncoef, int n) {
+    hh_context *c = p->ctx;
+    size_t b64 = (size_t)ncoef * n * 16, b32 = (size_t)ncoef * n * 8;
+    int rc = dev_alloc(c, &p->coef64, b64);
+    if (rc) return rc;
+    rc = dev_alloc(c, &p->coef32, b32);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(p->coef64, p->hcoef.data(), b64, hipMemcpyHostToDevice));
+    std::vector<float> h32(2 * (size_t)ncoef * n);
+    for (size_t i = 0; i < (size_t)ncoef * n; i++) {
+        h32[2 * i] = (float)p->hcoef[i].real();
+        h32[2 * i + 1] = (float)p->hcoef[i].imag();
+    }
+    HIPCHK(hipMemcpy(p->coef32, h32.data(), b32, hipMemcpyHostToDevice));
+    auto eig = [](int N, std::vector<double> &v) {
+        v.resize(N);
+        for (int k = 0; k < N; k++) v[k] = 2.0 * std::cos((k + 1) * M_PI / (N + 1));   // LU.c:12-14
+    };
+    std::vector<double> lx, ly;
+    eig(p->Nx, lx);
+    rc = dev_alloc(c, (void **)&p->lamx, lx.size() * 8);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(p->lamx, lx.data(), lx.size() * 8, hipMemcpyHostToDevice));
+    if (p->dim == 3) {
+        eig(p->Ny, ly);
+        rc = dev_alloc(c, (void **)&p->lamy, ly.size() * 8);
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(p->lamy, ly.data(), ly.size() * 8, hipMemcpyHostToDevice));
+    }
+    return HH_OK;
+}
+
+static int plan_check_dims(int dim, int Nx, int Ny, int Nz) {
+    if (Nx < 2 || Ny < 2 || (dim == 3 && Nz < 2))
+        return fail(HH_ERR_ARG, "grid sizes must be at least 2 (got %d x %d x %d)", Nx, Ny, Nz);
+    long big = 1L << 30;
+    if ((long)Nx * Ny * (dim == 3 ? Nz : 1) > 64 * big) return fail(HH_ERR_UNSUPPORTED, "grid too large");
+    int sys = dim == 2 ? Ny : Nz;
+    if (sys > 8192) return fail(HH_ERR_UNSUPPORTED, "system length %d above 8192", sys);
+    return HH_OK;
+}
+
+extern "C" int hh_plan2d_create(hh_context *ctx, int Nx, int Ny, const double *bp, const double *ap,
+                                const double *b, const double *a, const double *bm, const double *am, int precision,
+                                hh_plan **out) {
+    if (!ctx || !out || !bp || !ap || !b || !a || !bm || !am) return fail(HH_ERR_ARG, "hh_plan2d_create: null argument");
+    if (precision != HH_PREC_F64 && precision != HH_PREC_MIXED) return fail(HH_ERR_ARG, "unknown precision %d", precision);
+    int rc = plan_check_dims(2, Nx, Ny, 1);
+    if (rc) return rc;
+    if ((rc = set_device(ctx))) return rc;
+    hh_plan *p = new hh_plan();
+    p->ctx = ctx; p->dim = 2; p->Nx = Nx; p->Ny = Ny; p->Nz = 1; p->prec = precision;
+    p->hcoef.resize(6 * (size_t)Ny);
+    for (int q = 0; q < Ny; q++) {   // device band table order: bm, am, b, a, bp, ap
+        p->hcoef[6 * q + 0] = ld(bm, q); p->hcoef[6 * q + 1] = ld(am, q);
+        p->hcoef[6 * q + 2] = ld(b, q);  p->hcoef[6 * q + 3] = ld(a, q);
+        p->hcoef[6 * q + 4] = ld(bp, q); p->hcoef[6 * q + 5] = ld(ap, q);
+    }
+    if ((rc = upload_plan(p, 6, Ny))) { hh_plan_destroy(p); return rc; }
+    *out = p;
+    return HH_OK;
+}
+
+extern "C" int hh_plan3d_create(hh_context *ctx, int Nx, int Ny, int Nz, const double *const coef[12], int precision,
+                                hh_plan **out) {
+    if (!ctx || !out || !coef) return fail(HH_ERR_ARG, "hh_plan3d_create: null argument");
+    for (int q = 0; q < 12; q++)
+        if (!coef[q]) return fail(HH_ERR_ARG, "hh_plan3d_create: null coefficient array %d", q);
+    if (precision != HH_PREC_F64 && precision != HH_PREC_MIXED) return fail(HH_ERR_ARG, "unknown precision %d", precision);
+    int rc = plan_check_dims(3, Nx, Ny, Nz);
+    if (rc) return rc;
+    if ((rc = set_device(ctx))) return rc;
+    hh_plan *p = new hh_plan();
+    p->ctx = ctx; p->dim = 3; p->Nx = Nx; p->Ny = Ny; p->Nz = Nz; p->prec = precision;
+    p->hcoef.resize(12 * (size_t)Nz);
+    for (int l = 0; l < Nz; l++)
+        for (int q = 0; q < 12; q++) p->hcoef[12 * l + q] = ld(coef[q], l);
+    if ((rc = upload_plan(p, 12, Nz))) { hh_plan_destroy(p); return rc; }
+    *out = p;
+    return HH_OK;
+}
+
+extern "C" int hh_plan_destroy(hh_plan *p) {
+    if (!p) return HH_OK;
+    hh_context *c = p->ctx;
+    int nc = p->dim == 2 ? 6 : 12, n = p->dim == 2 ? p->Ny : p->Nz;
+    (void)hipStreamSynchronize(c->stream);
+    dev_free(c, p->coef64, (size_t)nc * n * 16);
+    dev_free(c, p->coef32, (size_t)nc * n * 8);
+    dev_free(c, p->lamx, (size_t)p->Nx * 8);
+    if (p->lamy) dev_free(c, p->lamy, (size_t)p->Ny * 8);
+    delete p;
+    return HH_OK;
+}
+
+extern "C" hh_context *hh_plan_context(const hh_plan *p) { return p ? p->ctx : nullptr; }
+
+extern "C" int hh_plan_dims(const hh_plan *p, int dims[3]) {
+    if (!p || !dims) return fail(HH_ERR_ARG, "hh_plan_dims: null argument");
+    dims[0] = p->Nx; dims[1] = p->Ny; dims[2] = p->Nz;
+    return p->dim;
+}
+
+extern "C" long hh_plan_field_size(const hh_plan *p) {
+    return p ? (long)p->Nx * p->Ny * (p->dim == 3 ? p->Nz : 1) : 0;
+}
+
+extern "C" size_t hh_plan_workspace_bytes(const hh_plan *p, int batch) {
+    if (!p || batch < 1) return 0;
+    size_t f = (size_t)hh_plan_field_size(p) * batch;
+    size_t ws = 0;
+    if (!plan_all_fft(p)) ws += f * 16;      // ping-pong buffer of the dense DST path
+    if (p->prec == HH_PREC_MIXED) ws += f * 8;   // FP32 correction
+    return ws;
+}
+
+extern "C" int hh_plan_set_refine(hh_plan *p, double tol, int max_iter) {
+    if (!p || tol < 0 || max_iter < 0) return fail(HH_ERR_ARG, "hh_plan_set_refine: bad arguments");
+    p->tol = tol;
+    p->max_iter = max_iter;
+    return HH_OK;
+}
+
+extern "C" int hh_plan_last_refine(const hh_plan *p, int *sweeps, double *rel) {
+    if (!p) return fail(HH_ERR_ARG, "null plan");
+    if (sweeps) *sweeps = p->last_sweeps;
+    if (rel) *rel = p->last_rel;
+    return HH_OK;
+}
+
+// ------------------------------------------------------------------ solve
+// Transform-solve-transform on one buffer.  Types: ti input, tc compute /
+// intermediate storage, to output (0 = FP64, 1 = FP32).  tri_comp is the
+// arithmetic of the tridiagonal elimination.
+static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const void *rhs, void *sol, void *ws,
+                      int batch) {
+    hh_context *c = p->ctx;
+    const int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz;
+    int rc;
+    hh::TriHelm t;
+    t.dim = p->dim; t.Nx = Nx; t.Ny = Ny; t.Nz = Nz;
+    t.field = hh_plan_field_size(p);
+    t.batch = batch;
+    t.coef = tri_comp == 0 ? p->coef64 : p->coef32;
+    t.lamx = p->lamx; t.lamy = p->lamy;
+    int n = p->dim == 2 ? Ny : Nz;
+    long nmodes = p->dim == 2 ? Nx : (long)Nx * Ny;
+    hh::tri_shape(n, nmodes, tri_comp == 0 && tc == 0 ? hh::F64 : hh::F32, &t.W, &t.P, &t.mmax);
+    if (tri_comp == 0 && tc == 1) {   // FP64 elimination on FP32 storage: FP64 register budget
+        hh::tri_shape(n, nmodes, hh::F64, &t.W, &t.P, &t.mmax);
+    }
+    auto tri = [&](void *buf) -> int {
+        TimeScope ts(c, 2, (double)t.field * batch * (tc ? 16 : 32));
+        hipError_t e = hh::launch_tri_helm(t, hh::DstTypes{tc, tri_comp, tc}, buf, buf, c->stream);
+        if (e != hipSuccess) return fail(HH_ERR_DEVICE, "tridiagonal launch failed: %s", hipGetErrorString(e));
+        return HH_OK;
+    };
+    const bool fft = plan_all_fft(p);
+    if (p->dim == 2) {
+        hh::Lines lx = rows_lines(Nx, (long)Ny * batch);
+        // intermediate buffer: sol (in place) on the FFT path; ws otherwise
+        void *mid = fft ? (tc == to ? sol : ws) : ws;
+        if (!mid) return fail(HH_ERR_ARG, "internal: missing workspace");
+        if ((rc = dst_pass(c, lx, ti, tc, tc, rhs, mid))) return rc;
+        if ((rc = tri(mid))) return rc;
+        return dst_pass(c, lx, tc, tc, to, mid, sol);
+    }
+    hh::Lines lx = rows_lines(Nx, (long)Ny * Nz * batch);
+    hh::Lines ly = cols_lines(Nx, Ny, (long)Nz * batch);
+    if (fft && tc == to) {
+        if ((rc = dst_pass(c, lx, ti, tc, tc, rhs, sol))) return rc;
+        if ((rc = dst_pass(c, ly, tc, tc, tc, sol, sol))) return rc;
+        if ((rc = tri(sol))) return rc;
+        if ((rc = dst_pass(c, ly, tc, tc, tc, sol, sol))) return rc;
+        return dst_pass(c, lx, tc, tc, to, sol, sol);
+    }
+    if (!ws) return fail(HH_ERR_ARG, "internal: missing workspace");
+    if (fft) {   // intermediate precision differs from the output: keep it in ws
+        if ((rc = dst_pass(c, lx, ti, tc, tc, rhs, ws))) return rc;
+        if ((rc = dst_pass(c, ly, tc, tc, tc, ws, ws))) return rc;
+        if ((rc = tri(ws))) return rc;
+        if ((rc = dst_pass(c, ly, tc, tc, tc, ws, ws))) return rc;
+        return dst_pass(c, lx, tc, tc, to, ws, sol);
+    }
+    if (tc != to || ti != tc) return fail(HH_ERR_UNSUPPORTED, "mixed precision needs power-of-two N+1 in 3D");
+    if ((rc = dst_pass(c, lx, ti, tc, tc, rhs, ws))) return rc;
+    if ((rc = dst_pass(c, ly, tc, tc, tc, ws, sol))) return rc;
+    if ((rc = tri(sol))) return rc;
+    if ((rc = dst_pass(c, ly, tc, tc, tc, sol, ws))) return rc;
+    return dst_pass(c, lx, tc, tc, to, ws, sol);
+}
+
+static int residual_into(hh_plan *p, const void *u, const void *F, void *res, int res_f32, int batch) {
+    TimeScope ts(p->ctx, 3, (double)hh_plan_field_size(p) * batch * (32 + (res_f32 ? 8 : 16)));
+    hipError_t e;
+    if (p->dim == 2) e = hh::launch_residual2d(p->Nx, p->Ny, batch, p->coef64, u, F, res, res_f32, p->ctx->stream);
+    else e = hh::launch_residual3d(p->Nx, p->Ny, p->Nz, batch, p->coef64, u, F, res, res_f32, p->ctx->stream);
+    if (e != hipSuccess) return fail(HH_ERR_DEVICE, "residual launch failed: %s", hipGetErrorString(e));
+    return HH_OK;
+}
+
+static int norms_sync(hh_context *c, long n, const void *x, int x_f32, const void *y, double out[2]) {
+    hipError_t e = hh::launch_norms(n, x, x_f32, y, c->d_norm, c->stream);
+    if (e != hipSuccess) return fail(HH_ERR_DEVICE, "norm launch failed: %s", hipGetErrorString(e));
+    HIPCHK(hipMemcpyAsync(out, c->d_norm, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return HH_OK;
+}
+
+extern "C" int hh_solve(hh_plan *p, const void *rhs, void *sol, int batch) {
+    if (!p || !rhs || !sol) return fail(HH_ERR_ARG, "hh_solve: null argument");
+    if (batch < 1) return fail(HH_ERR_ARG, "hh_solve: batch must be >= 1 (got %d)", batch);
+    hh_context *c = p->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = set_device(c);
+    if (rc) return rc;
+    size_t ws_bytes = hh_plan_workspace_bytes(p, batch);
+    if (ws_bytes && (rc = ensure_scratch(c, ws_bytes))) return rc;
+    const long n = hh_plan_field_size(p) * batch;
+    if (p->prec == HH_PREC_F64) {
+        void *ws = plan_all_fft(p) ? nullptr : c->scratch;
+        return solve_core(p, 0, 0, 0, 0, rhs, sol, ws, batch);
+    }
+    // mixed: FP32 transforms and storage, FP64 elimination, FP64 residual
+    // and update; the correction lives in FP32 scratch
+    void *dlt = c->scratch;
+    if (!plan_all_fft(p)) return fail(HH_ERR_UNSUPPORTED, "mixed precision needs power-of-two N+1");
+    if (rhs == sol) return fail(HH_ERR_ARG, "mixed precision solve needs d_sol != d_rhs");
+    if ((rc = solve_core(p, 0, 1, 0, 0, rhs, sol, dlt, batch))) return rc;   // u0 = A32^{-1} F
+    double nf[2], nr[2];
+    if ((rc = norms_sync(c, n, rhs, 0, nullptr, nf))) return rc;
+    const double fn = nf[0] > 0 ? nf[0] : 1.0;
+    int sweeps = 0;
+    double rel = 0;
+    for (;;) {
+        // r = A u - F with FP64 arithmetic, stored in FP32 (its norm only
+        // needs a few digits; the correction it drives is an FP32 solve)
+        if ((rc = residual_into(p, sol, rhs, dlt, 1, batch))) return rc;
+        if ((rc = norms_sync(c, n, dlt, 1, nullptr, nr))) return rc;
+        rel = nr[0] / fn;
+        if (rel <= p->tol || sweeps >= p->max_iter) break;
+        if ((rc = solve_core(p, 1, 1, 1, 0, dlt, dlt, dlt, batch))) return rc;   // A d = r
+        TimeScope ts(c, 3, (double)n * 40);
+        hipError_t e = hh::launch_axpy_update(n, sol, dlt, 1, -1.0, c->stream);    // u -= d
+        if (e != hipSuccess) return fail(HH_ERR_DEVICE, "update launch failed: %s", hipGetErrorString(e));
+        sweeps++;
+    }
+    p->last_sweeps = sweeps;
+    p->last_rel = rel;
+    return HH_OK;
+}
+
+extern "C" int hh_solve_host(hh_plan *p, const double *h_rhs, double *h_sol, int batch) {
+    if (!p || !h_rhs || !h_sol) return fail(HH_ERR_ARG, "hh_solve_host: null argument");
+    if (batch < 1) return fail(HH_ERR_ARG, "hh_solve_host: batch must be >= 1 (got %d)", batch);
+    hh_context *c = p->ctx;
+    int rc = set_device(c);
+    if (rc) return rc;
+    size_t bytes = (size_t)hh_plan_field_size(p) * batch * 16;
+    void *d_rhs = nullptr, *d_sol = nullptr;
+    if ((rc = dev_alloc(c, &d_rhs, bytes))) return rc;
+    if ((rc = dev_alloc(c, &d_sol, bytes))) { dev_free(c, d_rhs, bytes); return rc; }
+    hipError_t e = hipMemcpyAsync(d_rhs, h_rhs, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        rc = hh_solve(p, d_rhs, d_sol, batch);
+        if (rc == HH_OK) {
+            e = hipMemcpyAsync(h_sol, d_sol, bytes, hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        }
+    }
+    (void)hipStreamSynchronize(c->stream);
+    dev_free(c, d_rhs, bytes);
+    dev_free(c, d_sol, bytes);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(HH_ERR_DEVICE, "host transfer failed: %s", hipGetErrorString(e));
+    return HH_OK;
+}
+
+extern "C" int hh_residual(hh_plan *p, const void *d_sol, const void *d_rhs, void *d_res, int batch) {
+    if (!p || !d_sol || !d_rhs || !d_res) return fail(HH_ERR_ARG, "hh_residual: null argument");
+    if (batch < 1) return fail(HH_ERR_ARG, "hh_residual: batch must be >= 1");
+    int rc = set_device(p->ctx);
+    if (rc) return rc;
+    return residual_into(p, d_sol, d_rhs, d_res, 0, batch);
+}
+
+extern "C" int hh_dst_rows(hh_context *c, int N, long nrows, const void *in, void *out) {
+    if (!c || !in || !out || N < 1 || nrows < 0) return fail(HH_ERR_ARG, "hh_dst_rows: bad arguments");
+    if (nrows == 0) return HH_OK;
+    if (!hh::fft_supported(N) && N > 4096)
+        return fail(HH_ERR_UNSUPPORTED, "DST length %d too large for the dense path", N);
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = set_device(c);
+    if (rc) return rc;
+    hh::Lines l = rows_lines(N, nrows);
+    if (!hh::fft_supported(N) && in == out) {   // dense path is out of place
+        if ((rc = ensure_scratch(c, (size_t)N * nrows * 16))) return rc;
+        HIPCHK(hipMemcpyAsync(c->scratch, in, (size_t)N * nrows * 16, hipMemcpyDeviceToDevice, c->stream));
+        return dst_pass(c, l, 0, 0, 0, c->scratch, out);
+    }
+    return dst_pass(c, l, 0, 0, 0, in, out);
+}
+
+extern "C" int hh_dst2d_layers(hh_context *c, int Nx, int Ny, long nlayers, const void *in, void *out) {
+    if (!c || !in || !out || Nx < 1 || Ny < 1 || nlayers < 0) return fail(HH_ERR_ARG, "hh_dst2d_layers: bad arguments");
+    if (nlayers == 0) return HH_OK;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = set_device(c);
+    if (rc) return rc;
+    hh::Lines lx = rows_lines(Nx, (long)Ny * nlayers), ly = cols_lines(Nx, Ny, nlayers);
+    bool fx = hh::fft_supported(Nx), fy = hh::fft_supported(Ny);
+    size_t bytes = (size_t)Nx * Ny * nlayers * 16;
+    if (fx && fy) {
+        if ((rc = dst_pass(c, lx, 0, 0, 0, in, out))) return rc;
+        return dst_pass(c, ly, 0, 0, 0, out, out);
+    }
+    if ((rc = ensure_scratch(c, bytes))) return rc;
+    if ((rc = dst_pass(c, lx, 0, 0, 0, in, c->scratch))) return rc;
+    return dst_pass(c, ly, 0, 0, 0, c->scratch, out);
+}
+
+// compact (k^2)_y (2D_Program/partial_derivatives/4th_order_first_derivative.c:4-34)
+static void k2_y_compact(int Ny, double hy, const double *k2e, const double *k2yb, std::vector<cd> &k2y) {
+    k2y.resize(Ny);
+    for (int j = 0; j < Ny; j++) k2y[j] = 3.0 * (ld(k2e, j + 2) - ld(k2e, j)) / hy;
+    k2y[0] -= ld(k2yb, 0);
+    k2y[Ny - 1] -= ld(k2yb, 1);
+    compact_line(k2y, 4.0);
+}
+
+extern "C" int hh_rhs2d(hh_plan *p, int order, double hx, double hy, const void *f_e, const void *u_e,
+                        const void *fy_b, const void *fxx_b, const void *fyy_b, const double *k2e, const double *k2yb,
+                        void *F, int batch) {
+    if (!p || p->dim != 2 || !f_e || !u_e || !k2e || !F || batch < 1) return fail(HH_ERR_ARG, "hh_rhs2d: bad arguments");
+    if (order != 2 && order != 4 && order != 6) return fail(HH_ERR_ARG, "order must be 2, 4 or 6 (got %d)", order);
+    if (order == 6 && (!fy_b || !fxx_b || !fyy_b)) return fail(HH_ERR_ARG, "hh_rhs2d: sixth order needs boundary data");
+    hh_context *c = p->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = set_device(c);
+    if (rc) return rc;
+    const int Nx = p->Nx, Ny = p->Ny;
+    const long XY = (long)(Nx + 2) * (Ny + 2), wstride = 2 * XY + (long)Nx * Ny;
+    // per-row k^2 and compact (k^2)_y, uploaded after the field work area
+    size_t wbytes = order == 6 ? (size_t)wstride * batch * 16 : 0;
+    size_t tbytes = (size_t)2 * Ny * 16;
+    if ((rc = ensure_scratch(c, wbytes + tbytes))) return rc;
+    std::vector<cd> tab(2 * (size_t)Ny);
+    std::vector<cd> k2y;
+    if (order == 6) k2_y_compact(Ny, hy, k2e, k2yb, k2y);
+    for (int j = 0; j < Ny; j++) {
+        tab[j] = ld(k2e, j + 1);
+        tab[Ny + j] = order == 6 ? k2y[j] : cd(0);
+    }
+    char *tb = (char *)c->scratch + wbytes;
+    HIPCHK(hipMemcpyAsync(tb, tab.data(), tbytes, hipMemcpyHostToDevice, c->stream));
+    hh::Rhs2D r;
+    r.order = order; r.Nx = Nx; r.Ny = Ny; r.batch = batch; r.hx = hx; r.hy = hy;
+    r.f_e = f_e; r.u_e = u_e; r.fy_b = fy_b; r.fxx_b = fxx_b; r.fyy_b = fyy_b;
+    r.k2c = tb; r.k2y = tb + (size_t)Ny * 16; r.coef = p->coef64; r.work = c->scratch;
+    TimeScope ts(c, 4, (double)Nx * Ny * batch * 32);
+    hipError_t e = hh::launch_rhs2d(r, F, c->stream);
+    if (e != hipSuccess) return fail(HH_ERR_DEVICE, "rhs2d launch failed: %s", hipGetErrorString(e));
+    // the host table must outlive the asynchronous copy
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return HH_OK;
+}
+
+extern "C" int hh_rhs3d(hh_plan *p, int order, double hx, double hy, double hz, const void *f_e, const void *u_e,
+                        const void *fxx_f, const void *fyy_f, const void *fzz_f, const double *k2e, const double *k2d,
+                        void *F, int batch) {
+    if (!p || p->dim != 3 || !f_e || !u_e || !k2e || !F || batch < 1) return fail(HH_ERR_ARG, "hh_rhs3d: bad arguments");
+    if (order != 2 && order != 4 && order != 6) return fail(HH_ERR_ARG, "order must be 2, 4 or 6 (got %d)", order);
+    if (order == 6 && (!fxx_f || !fyy_f || !fzz_f)) return fail(HH_ERR_ARG, "hh_rhs3d: sixth order needs face data");
+    hh_context *c = p->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = set_device(c);
+    if (rc) return rc;
+    const int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz;
+    const long E = (long)(Nx + 2) * (Ny + 2) * (Nz + 2);
+    size_t wbytes = order == 6 ? (size_t)3 * E * batch * 16 : 0;
+    size_t tbytes = (size_t)(2 * Nz + 2) * 16;
+    if ((rc = ensure_scratch(c, wbytes + tbytes))) return rc;
+    // k^2 on the extended z grid and compact (k^2)_z (derivative_kz_4th_order)
+    std::vector<cd> tab(2 * (size_t)Nz + 2);
+    for (int l = 0; l < Nz + 2; l++) tab[l] = ld(k2e, l);
+    std::vector<cd> kz(Nz);
+    for (int l = 0; l < Nz; l++) kz[l] = 3.0 * (ld(k2e, l + 2) - ld(k2e, l)) / hz;
+    kz[0] -= ld(k2d, 0);
+    kz[Nz - 1] -= ld(k2d, 1);
+    compact_line(kz, 4.0);
+    for (int l = 0; l < Nz; l++) tab[Nz + 2 + l] = kz[l];
+    char *tb = (char *)c->scratch + wbytes;
+    HIPCHK(hipMemcpyAsync(tb, tab.data(), tbytes, hipMemcpyHostToDevice, c->stream));
+    hh::Rhs3D r;
+    r.order = order; r.Nx = Nx; r.Ny = Ny; r.Nz = Nz; r.batch = batch; r.hx = hx; r.hy = hy; r.hz = hz;
+    r.f_e = f_e; r.u_e = u_e; r.fxx_f = fxx_f; r.fyy_f = fyy_f; r.fzz_f = fzz_f;
+    r.k2e = tb; r.kz = tb + (size_t)(Nz + 2) * 16; r.coef = p->coef64; r.work = c->scratch;
+    TimeScope ts(c, 4, (double)Nx * Ny * Nz * batch * 32);
+    hipError_t e = hh::launch_rhs3d(r, F, c->stream);
+    if (e != hipSuccess) return fail(HH_ERR_DEVICE, "rhs3d launch failed: %s", hipGetErrorString(e));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return HH_OK;
+}
+
+extern "C" int hh_norms(hh_context *c, long n, const void *x, const void *y, double out[2]) {
+    if (!c || !x || !out || n < 0) return fail(HH_ERR_ARG, "hh_norms: bad arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = set_device(c);
+    if (rc) return rc;
+    return norms_sync(c, n, x, 0, y, out);
+}

@@ -1,0 +1,107 @@
+// Internal launch interface between the C-ABI layer and the kernel files.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace hh {
+
+enum Prec { F64 = 0, F32 = 1 };
+
+// Element types of a DST pass: input, compute, output (0 = double, 1 = float)
+struct DstTypes {
+    int in, comp, out;
+};
+
+// Lines of a DST-I pass.  Line l = (plane p, column c), p = l / ncols,
+// c = l % ncols; element q of the line sits at
+//   p*plane_stride + c*col_stride + q*elem_stride.
+struct Lines {
+    int N;              // line length
+    long ncols;         // lines per plane
+    long nplanes;       // planes (batch folded in)
+    long plane_stride;
+    long col_stride;
+    long elem_stride;
+};
+
+bool fft_supported(int N);  // N+1 a power of two, 2(N+1) <= HH_FFT_LMAX
+
+// Orthonormal DST-I of every line.  FFT path (in == out allowed) when
+// fft_supported(N), else the dense-matrix path (in != out required).
+// tw: device twiddle table e^{-2 pi i q/L}, L = 2(N+1), in the compute type.
+// S: device DST matrix (N*N, compute type) for the dense path.
+hipError_t launch_dst(const Lines &ln, DstTypes ty, const void *in, void *out,
+                      const void *tw, const void *S, hipStream_t st);
+
+// Per-mode tridiagonal solves of the Helmholtz operator along the last axis.
+struct TriHelm {
+    int dim;            // 2 or 3
+    int Nx, Ny, Nz;     // 2D: Nz = 1, system length Ny; 3D: length Nz
+    long field;         // elements per right-hand side
+    int batch;
+    const void *coef;   // 2D: [q][6] (bm,am,b,a,bp,ap); 3D: [q][12] (a..dm)
+    const double *lamx, *lamy;   // eigenvalues 2cos(k pi/(N+1))
+    int W, P, mmax;     // block shape (modes per block, chunks per mode, rows per chunk)
+};
+hipError_t launch_tri_helm(const TriHelm &t, DstTypes ty, const void *in, void *out, hipStream_t st);
+
+// Compact fourth-order derivative line solves (band 1, diag, 1) over an
+// extended-grid field; see CompactProb in hh_tri.hip for the index maps.
+struct Compact {
+    int n, first;
+    double diag, h;
+    const void *f;
+    long field_in, mA, orig, sA, sB, eS;
+    const void *lo, *hi;
+    long field_bnd, bo, bA, bB;
+    void *out;
+    long field_out, oorig, oA, oB, oS;
+    long nmodes;
+    int batch;
+};
+hipError_t launch_compact(const Compact &cd, hipStream_t st);
+
+// Choose a tridiagonal block shape for a system of length n and nmodes modes.
+void tri_shape(int n, long nmodes, int prec, int *W, int *P, int *mmax);
+
+// Compact-stencil right-hand-side assembly (2D_Program/main_functions/rhs.c,
+// 3D_Program/main_functions/rhs.c and the partial derivatives).
+struct Rhs2D {
+    int order, Nx, Ny, batch;
+    double hx, hy;
+    const void *f_e;     // (Nx+2)(Ny+2) per rhs
+    const void *u_e;     // (Nx+2)(Ny+2) per rhs, Dirichlet data
+    const void *fy_b;    // 2*Nx per rhs
+    const void *fxx_b;   // 2(Nx+2)+2(Ny+2) per rhs (ring)
+    const void *fyy_b;   // ring
+    const void *k2c;     // Ny: k^2 at interior rows
+    const void *k2y;     // Ny: compact (k^2)_y at interior rows
+    const void *coef;    // Ny x 6
+    void *work;          // 3 (Nx+2)(Ny+2)+Nx Ny per rhs
+};
+hipError_t launch_rhs2d(const Rhs2D &r, void *F, hipStream_t st);
+
+struct Rhs3D {
+    int order, Nx, Ny, Nz, batch;
+    double hx, hy, hz;
+    const void *f_e, *u_e;        // extended (Nx+2)(Ny+2)(Nz+2) per rhs
+    const void *fxx_f, *fyy_f, *fzz_f;  // face arrays
+    const void *k2e;              // Nz+2
+    const void *kz;               // Nz: compact (k^2)_z
+    const void *coef;             // Nz x 12
+    void *work;                   // 3 extended fields per rhs
+};
+hipError_t launch_rhs3d(const Rhs3D &r, void *F, hipStream_t st);
+
+// res = A u - F (the reference residual.c), and the fused refinement pieces.
+hipError_t launch_residual2d(int Nx, int Ny, int batch, const void *coef, const void *u,
+                             const void *F, void *res, int res_f32, hipStream_t st);
+hipError_t launch_residual3d(int Nx, int Ny, int Nz, int batch, const void *coef, const void *u,
+                             const void *F, void *res, int res_f32, hipStream_t st);
+// u += s*c (c in FP32 or FP64)
+hipError_t launch_axpy_update(long n, void *u, const void *c, int c_f32, double s, hipStream_t st);
+// out[0] = max |x|, out[1] = max |x - y| (y may be null); out is device memory
+hipError_t launch_norms(long n, const void *x, int x_f32, const void *y, double *out, hipStream_t st);
+hipError_t launch_convert(long n, const void *in, int in_f32, void *out, int out_f32, hipStream_t st);
+
+}  // namespace hh

@@ -1,0 +1,357 @@
+// Per-mode tridiagonal solves for gfx950 (the reference's "Middle stuff":
+// 2D_Program/solver_functions/solver.c:71-86 with the factors of
+// 2D_Program/main_functions/LU.c:3-46; 3D_Program/solver_functions/
+// solver.c:356-369 / cuda_kernels.cu:208-249 with 3D_Program/main_functions/
+// LU.c:4-81).
+//
+// Mode m owns the system  sub_q x_{q-1} + dia_q x_q + sup_q x_{q+1} = d_q,
+// q = 0..n-1, along the last axis (y in 2D, z in 3D).  The bands are
+// rebuilt on the fly from the O(n) stencil coefficient vectors and the
+// closed-form eigenvalues 2cos(k pi/(N+1)); nothing of size N^2 or N^3 is
+// stored (the reference streams 48 B/DOF of L, U, Up).
+//
+// Block = W neighbouring modes x P chunks.  Thread (w, c) owns rows
+// [s, e] of mode w in registers (row-major layout is kept: lanes w are
+// adjacent in memory, so each load instruction moves W*16-byte segments).
+//   1. downward sweep: row q in the form  a~_q x_s + x_q + c~_q x_{q+1} = d~_q
+//   2. upward sweep:   row q in the form  a^_q x_s + x_q + c^_q x_e = d^_q
+//   3. the chunk's first and last rows give two rows of a reduced
+//      tridiagonal system of size 2P in (x_s^0, x_e^0, x_s^1, ...), solved by
+//      parallel cyclic reduction in LDS with normalised rows;
+//   4. x_q = d^_q - a^_q x_s - c^_q x_e, store.
+// HBM is read once and written once; the solve may run in place.
+#include "hh_common.h"
+#include "hh_internal.h"
+
+namespace hh {
+
+// Helmholtz mode bands.  coef is an array of complex rows: 2D [q][6] =
+// (bm, am, b, a, bp, ap) with sub = bm lam + am etc.; 3D [q][12] =
+// (a, b, c, d, ap, bp, cp, dp, am, bm, cm, dm) with
+// dia = a lx ly + b lx + c ly + d.
+template <typename T> struct HelmBands {
+    int dim, n, Nx;
+    const C2<T> *coef;
+    const double *lamx, *lamy;
+    struct Mode { T l0, l1, l2; };
+    HH_DEV Mode mode(long m) const {
+        Mode md;
+        if (dim == 2) {
+            md.l0 = (T)lamx[m]; md.l1 = 0; md.l2 = 0;
+        } else {
+            long i = m % Nx, j = m / Nx;
+            T lx = (T)lamx[i], ly = (T)lamy[j];
+            md.l0 = lx * ly; md.l1 = lx; md.l2 = ly;
+        }
+        return md;
+    }
+    HH_DEV void band(const Mode &md, int q, C2<T> &sub, C2<T> &dia, C2<T> &sup) const {
+        if (dim == 2) {
+            const C2<T> *c = coef + 6 * (long)q;
+            T l = md.l0;
+            sub = mk<T>(c[0].x * l + c[1].x, c[0].y * l + c[1].y);
+            dia = mk<T>(c[2].x * l + c[3].x, c[2].y * l + c[3].y);
+            sup = mk<T>(c[4].x * l + c[5].x, c[4].y * l + c[5].y);
+        } else {
+            const C2<T> *c = coef + 12 * (long)q;
+            T a = md.l0, b = md.l1, g = md.l2;
+            dia = mk<T>(c[0].x * a + c[1].x * b + c[2].x * g + c[3].x, c[0].y * a + c[1].y * b + c[2].y * g + c[3].y);
+            sup = mk<T>(c[4].x * a + c[5].x * b + c[6].x * g + c[7].x, c[4].y * a + c[5].y * b + c[6].y * g + c[7].y);
+            sub = mk<T>(c[8].x * a + c[9].x * b + c[10].x * g + c[11].x, c[8].y * a + c[9].y * b + c[10].y * g + c[11].y);
+        }
+        if (q == 0) sub = mk<T>(0, 0);
+        if (q == n - 1) sup = mk<T>(0, 0);
+    }
+};
+
+// Helmholtz problem: bands from HelmBands, right-hand side and solution in
+// the row-major field (element (b, m, q) at b*field + m + q*S).
+template <typename T, typename Ti, typename To> struct HelmProb {
+    HelmBands<T> bands;
+    const C2<Ti> *in;
+    C2<To> *out;
+    long field, S;
+    struct Mode { typename HelmBands<T>::Mode bm; long m; };
+    HH_DEV int n() const { return bands.n; }
+    HH_DEV Mode mode(long m) const { Mode md; md.bm = bands.mode(m); md.m = m; return md; }
+    HH_DEV void band(const Mode &md, int q, C2<T> &a, C2<T> &b, C2<T> &c) const { bands.band(md.bm, q, a, b, c); }
+    HH_DEV C2<T> rhs(int b, const Mode &md, int q) const { return cconv<T, Ti>(in[b * field + md.m + (long)q * S]); }
+    HH_DEV void store(int b, const Mode &md, int q, C2<T> x) const {
+        out[b * field + md.m + (long)q * S] = cconv<To, T>(x);
+    }
+};
+
+// Compact fourth-order derivative of a field sampled on the extended grid
+// (2D_Program/partial_derivatives/4th_order_*.c, 3D_Program/
+// partial_derivatives/4th_order_partials.c:94-294): band (1, diag, 1),
+//   second derivative (diag 10): rhs = 12 (f_{q+1} - 2 f_q + f_{q-1}) / h^2
+//   first derivative  (diag 4):  rhs = 3 (f_{q+1} - f_{q-1}) / h
+// minus the analytic boundary value at q = 0 and q = n-1.  Mode m maps to
+// the extended-grid line origin + (m % mA) sA + (m / mA) sB; element q sits
+// at (q+1) eS along it.  The result goes to oorig + (m%mA) oA + (m/mA) oB + q oS.
+template <typename T> struct CompactProb {
+    int nn, first;
+    T diag, h;
+    const C2<T> *f;
+    long field_in, field_out, field_bnd;
+    long mA, orig, sA, sB, eS;
+    const C2<T> *lo, *hi;   // boundary values, indexed bo + (m%mA) bA + (m/mA) bB
+    long bo, bA, bB;
+    C2<T> *out;
+    long oorig, oA, oB, oS;
+    struct Mode { long in, bnd, out; };
+    HH_DEV int n() const { return nn; }
+    HH_DEV Mode mode(long m) const {
+        long a = m % mA, b = m / mA;
+        Mode md;
+        md.in = orig + a * sA + b * sB;
+        md.bnd = bo + a * bA + b * bB;
+        md.out = oorig + a * oA + b * oB;
+        return md;
+    }
+    HH_DEV void band(const Mode &, int q, C2<T> &a, C2<T> &b, C2<T> &c) const {
+        a = mk<T>(q > 0 ? (T)1 : (T)0, 0);
+        b = mk<T>(diag, 0);
+        c = mk<T>(q < nn - 1 ? (T)1 : (T)0, 0);
+    }
+    HH_DEV C2<T> rhs(int bt, const Mode &md, int q) const {
+        const C2<T> *g = f + bt * field_in + md.in + (long)(q + 1) * eS;
+        C2<T> v;
+        if (first) v = cscale<T>(csub<T>(g[eS], g[-eS]), (T)3 / h);
+        else v = cscale<T>(cadd<T>(csub<T>(g[eS], cscale<T>(g[0], (T)2)), g[-eS]), (T)12 / (h * h));
+        if (q == 0) v = csub<T>(v, lo[bt * field_bnd + md.bnd]);
+        if (q == nn - 1) v = csub<T>(v, hi[bt * field_bnd + md.bnd]);
+        return v;
+    }
+    HH_DEV void store(int bt, const Mode &md, int q, C2<T> x) const {
+        out[bt * field_out + md.out + (long)q * oS] = x;
+    }
+};
+
+// Blocks are dealt round-robin to the 8 XCDs (block b runs on XCD b % 8).
+// Neighbouring mode tiles share cache lines when W*sizeof(complex) < 128,
+// so tiles are renumbered to run consecutively on one XCD and its L2.
+HH_DEV long xcd_tile(long b, long G) {
+    const long G8 = G & ~7L;
+    if (b >= G8) return b;
+    return (b & 7) * (G8 >> 3) + (b >> 3);
+}
+
+template <typename T, int MMAX, int NT, class Prob>
+__global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C2<T> *rA = reinterpret_cast<C2<T> *>(smem_raw);
+    C2<T> *rC = rA + 2 * P * W;
+    C2<T> *rD = rC + 2 * P * W;
+    const int n = prob.n();
+    const int w = threadIdx.x % W, c = threadIdx.x / W;
+    const long m = xcd_tile(blockIdx.x, gridDim.x) * W + w;
+    const bool valid = (c < P) && (m < nmodes);
+    const int bt = blockIdx.y;
+    const int s = (int)(((long)c * n) / P), e = (int)(((long)(c + 1) * n) / P) - 1;
+    const int len = e - s + 1;
+    const auto md = prob.mode(valid ? m : 0);
+
+    C2<T> ab[MMAX - 1], cb[MMAX - 1], db[MMAX - 1];
+    C2<T> sub_s = mk<T>(0, 0), dia_s = mk<T>(1, 0), sup_s = mk<T>(0, 0), d_s = mk<T>(0, 0);
+    C2<T> pa = mk<T>(0, 0), pc = mk<T>(0, 0), pd = mk<T>(0, 0);
+    if (valid) {
+        prob.band(md, s, sub_s, dia_s, sup_s);
+        d_s = prob.rhs(bt, md, s);
+    }
+    // 1. downward sweep over rows s+1..e
+#pragma unroll
+    for (int t = 0; t < MMAX - 1; t++) {
+        if (t < len - 1) {
+            const int q = s + 1 + t;
+            C2<T> a = mk<T>(0, 0), b = mk<T>(1, 0), cc = mk<T>(0, 0), d = mk<T>(0, 0);
+            if (valid) {
+                prob.band(md, q, a, b, cc);
+                d = prob.rhs(bt, md, q);
+            }
+            C2<T> den = (t == 0) ? b : cfms<T>(a, pc, b);
+            C2<T> r = crcp<T>(den);
+            C2<T> na = (t == 0) ? cmul<T>(a, r) : cneg<T>(cmul<T>(cmul<T>(a, pa), r));
+            C2<T> nd = (t == 0) ? cmul<T>(d, r) : cmul<T>(cfms<T>(a, pd, d), r);
+            C2<T> nc = cmul<T>(cc, r);
+            ab[t] = na; cb[t] = nc; db[t] = nd;
+            pa = na; pc = nc; pd = nd;
+        }
+    }
+    // reduced row 2c+1 (x_e): pa x_s + x_e + pc x_{e+1} = pd  (already normalised)
+    C2<T> A1 = pa, C1 = pc, D1 = pd;
+    // 2. upward sweep over rows e-2..s+1 (row e-1 keeps its downward form)
+#pragma unroll
+    for (int t = MMAX - 3; t >= 0; t--) {
+        if (t <= len - 4) {
+            C2<T> cbt = cb[t];
+            ab[t] = cfms<T>(cbt, ab[t + 1], ab[t]);
+            db[t] = cfms<T>(cbt, db[t + 1], db[t]);
+            cb[t] = cneg<T>(cmul<T>(cbt, cb[t + 1]));
+        }
+    }
+    // reduced row 2c (x_s): sub_s x_{s-1} + B x_s + C x_e = D
+    C2<T> B0, C0, D0;
+    if (len >= 3) {
+        B0 = cfms<T>(sup_s, ab[0], dia_s);
+        C0 = cneg<T>(cmul<T>(sup_s, cb[0]));
+        D0 = cfms<T>(sup_s, db[0], d_s);
+    } else {
+        B0 = dia_s; C0 = sup_s; D0 = d_s;
+    }
+    {
+        C2<T> r = crcp<T>(B0);
+        B0 = cmul<T>(sub_s, r);  // reuse B0 as the normalised sub coefficient A0
+        C0 = cmul<T>(C0, r);
+        D0 = cmul<T>(D0, r);
+    }
+    C2<T> A0 = B0;
+    if (!valid) {  // identity rows for idle threads and padding modes
+        A0 = C0 = A1 = C1 = mk<T>(0, 0);
+        D0 = D1 = mk<T>(0, 0);
+    }
+    // 3. PCR on the reduced system, rows r0 = 2c, r1 = 2c+1, layout [row][W]
+    const int R = 2 * P;
+    const int r0 = 2 * c, r1 = 2 * c + 1;
+    if (c < P) {
+        rA[r0 * W + w] = A0; rC[r0 * W + w] = C0; rD[r0 * W + w] = D0;
+        rA[r1 * W + w] = A1; rC[r1 * W + w] = C1; rD[r1 * W + w] = D1;
+    }
+    for (int dd = 1; dd < R; dd <<= 1) {
+        __syncthreads();
+        C2<T> z = mk<T>(0, 0);
+        C2<T> aL0 = z, cL0 = z, dL0 = z, aU0 = z, cU0 = z, dU0 = z;
+        C2<T> aL1 = z, cL1 = z, dL1 = z, aU1 = z, cU1 = z, dU1 = z;
+        if (c < P) {
+            int q;
+            q = r0 - dd; if (q >= 0) { aL0 = rA[q * W + w]; cL0 = rC[q * W + w]; dL0 = rD[q * W + w]; }
+            q = r0 + dd; if (q < R) { aU0 = rA[q * W + w]; cU0 = rC[q * W + w]; dU0 = rD[q * W + w]; }
+            q = r1 - dd; if (q >= 0) { aL1 = rA[q * W + w]; cL1 = rC[q * W + w]; dL1 = rD[q * W + w]; }
+            q = r1 + dd; if (q < R) { aU1 = rA[q * W + w]; cU1 = rC[q * W + w]; dU1 = rD[q * W + w]; }
+        }
+        __syncthreads();
+        if (c < P) {
+            // row: A x_{r-d} + x_r + C x_{r+d} = D
+            C2<T> b0 = csub<T>(mk<T>(1, 0), cadd<T>(cmul<T>(A0, cL0), cmul<T>(C0, aU0)));
+            C2<T> na0 = cneg<T>(cmul<T>(A0, aL0)), nc0 = cneg<T>(cmul<T>(C0, cU0));
+            C2<T> nd0 = csub<T>(D0, cadd<T>(cmul<T>(A0, dL0), cmul<T>(C0, dU0)));
+            C2<T> b1 = csub<T>(mk<T>(1, 0), cadd<T>(cmul<T>(A1, cL1), cmul<T>(C1, aU1)));
+            C2<T> na1 = cneg<T>(cmul<T>(A1, aL1)), nc1 = cneg<T>(cmul<T>(C1, cU1));
+            C2<T> nd1 = csub<T>(D1, cadd<T>(cmul<T>(A1, dL1), cmul<T>(C1, dU1)));
+            C2<T> ib0 = crcp<T>(b0), ib1 = crcp<T>(b1);
+            A0 = cmul<T>(na0, ib0); C0 = cmul<T>(nc0, ib0); D0 = cmul<T>(nd0, ib0);
+            A1 = cmul<T>(na1, ib1); C1 = cmul<T>(nc1, ib1); D1 = cmul<T>(nd1, ib1);
+            rA[r0 * W + w] = A0; rC[r0 * W + w] = C0; rD[r0 * W + w] = D0;
+            rA[r1 * W + w] = A1; rC[r1 * W + w] = C1; rD[r1 * W + w] = D1;
+        }
+    }
+    if (!valid) return;
+    const C2<T> xs = D0, xe = D1;
+    // 4. back-substitution of the interior and store
+    prob.store(bt, md, s, xs);
+#pragma unroll
+    for (int t = 0; t < MMAX - 2; t++) {
+        if (t < len - 2) {
+            C2<T> x = csub<T>(db[t], cadd<T>(cmul<T>(ab[t], xs), cmul<T>(cb[t], xe)));
+            prob.store(bt, md, s + 1 + t, x);
+        }
+    }
+    prob.store(bt, md, e, xe);
+}
+
+template <typename T, int MM, class Prob>
+static hipError_t launch_tri_prob(const Prob &pb, int n, long nmodes, int batch, int W, int P, hipStream_t st) {
+    if (n < 2) return hipErrorInvalidValue;
+    // every chunk needs at least two rows, at most MM rows
+    if (P < 1 || P > n / 2 || (n + P - 1) / P > MM || W * P > 1024) return hipErrorInvalidValue;
+    int threads = ((W * P + 63) / 64) * 64;
+    size_t shm = sizeof(C2<T>) * 3 * 2 * (size_t)P * W;
+    // 512-thread bound: 256 VGPRs per lane, no spills of the chunk registers
+    auto kfn = threads <= 512 ? k_tri<T, MM, 512, Prob> : k_tri<T, MM, 1024, Prob>;
+    if (shm > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return e;
+    }
+    dim3 grid((unsigned)((nmodes + W - 1) / W), (unsigned)batch);
+    hipLaunchKernelGGL(kfn, grid, dim3(threads), shm, st, pb, nmodes, W, P);
+    return hipGetLastError();
+}
+
+void tri_shape(int n, long nmodes, int prec, int *W, int *P, int *mmax) {
+    // rows per thread: 8 (FP64) keeps the three per-row register arrays
+    // within 128 VGPRs for 1024-thread blocks; FP32 affords 16.
+    int mm = prec == F64 ? (n > 1024 ? 16 : 8) : 16;
+    if (n <= 16) mm = 4;
+    int p = (n + mm - 1) / mm;
+    if (p > n / 2) p = n / 2;
+    if (p < 1) p = 1;
+    // modes per block: as many as fit the thread budget (512 threads for
+    // FP64 so that the chunk registers do not spill), at most 16
+    int w = (prec == F64 ? 512 : 1024) / p;
+    if (w > 16) w = 16;
+    int pw = 1;
+    while (pw * 2 <= w) pw *= 2;
+    w = pw;
+    while (w > 1 && w / 2 >= nmodes) w /= 2;
+    *W = w; *P = p; *mmax = mm;
+}
+
+template <typename T, typename Ti, typename To, int MM>
+static hipError_t launch_tri_t(const TriHelm &t, const void *in, void *out, hipStream_t st) {
+    HelmProb<T, Ti, To> pb;
+    pb.bands.dim = t.dim;
+    pb.bands.Nx = t.Nx;
+    pb.bands.coef = (const C2<T> *)t.coef;
+    pb.bands.lamx = t.lamx;
+    pb.bands.lamy = t.lamy;
+    pb.in = (const C2<Ti> *)in;
+    pb.out = (C2<To> *)out;
+    pb.field = t.field;
+    long nmodes;
+    if (t.dim == 2) { pb.bands.n = t.Ny; nmodes = t.Nx; pb.S = t.Nx; }
+    else { pb.bands.n = t.Nz; nmodes = (long)t.Nx * t.Ny; pb.S = (long)t.Nx * t.Ny; }
+    return launch_tri_prob<T, MM>(pb, pb.bands.n, nmodes, t.batch, t.W, t.P, st);
+}
+
+template <typename T, typename Ti, typename To>
+static hipError_t launch_tri_mm(const TriHelm &t, const void *in, void *out, hipStream_t st) {
+    switch (t.mmax) {
+    case 4: return launch_tri_t<T, Ti, To, 4>(t, in, out, st);
+    case 8: return launch_tri_t<T, Ti, To, 8>(t, in, out, st);
+    case 16: return launch_tri_t<T, Ti, To, 16>(t, in, out, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_compact(const Compact &cd, hipStream_t st) {
+    CompactProb<double> pb;
+    pb.nn = cd.n;
+    pb.first = cd.first;
+    pb.diag = cd.diag;
+    pb.h = cd.h;
+    pb.f = (const double2 *)cd.f;
+    pb.field_in = cd.field_in;
+    pb.field_out = cd.field_out;
+    pb.field_bnd = cd.field_bnd;
+    pb.mA = cd.mA; pb.orig = cd.orig; pb.sA = cd.sA; pb.sB = cd.sB; pb.eS = cd.eS;
+    pb.lo = (const double2 *)cd.lo; pb.hi = (const double2 *)cd.hi;
+    pb.bo = cd.bo; pb.bA = cd.bA; pb.bB = cd.bB;
+    pb.out = (double2 *)cd.out;
+    pb.oorig = cd.oorig; pb.oA = cd.oA; pb.oB = cd.oB; pb.oS = cd.oS;
+    int W, P, mm;
+    tri_shape(cd.n, cd.nmodes, F64, &W, &P, &mm);
+    switch (mm) {
+    case 4: return launch_tri_prob<double, 4>(pb, cd.n, cd.nmodes, cd.batch, W, P, st);
+    case 8: return launch_tri_prob<double, 8>(pb, cd.n, cd.nmodes, cd.batch, W, P, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_tri_helm(const TriHelm &t, DstTypes ty, const void *in, void *out, hipStream_t st) {
+    if (ty.comp == 0 && ty.in == 0 && ty.out == 0) return launch_tri_mm<double, double, double>(t, in, out, st);
+    if (ty.comp == 1 && ty.in == 1 && ty.out == 1) return launch_tri_mm<float, float, float>(t, in, out, st);
+    if (ty.comp == 0 && ty.in == 1 && ty.out == 1) return launch_tri_mm<double, float, float>(t, in, out, st);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace hh

@@ -1,0 +1,300 @@
+"""Python host interface of the MI355X compact-Helmholtz solver.
+
+Mirrors the reference's host pipeline (2D_Program/main_functions/main.c:4-46,
+3D_Program/main_functions/main.c:4-39): coefficients -> rhs -> solver ->
+residual -> normInf, with the same row-major complex layouts, over the C ABI
+of libhh.so (include/hh_solver.h).  PyTorch is used only to hold device
+memory and to supply the stream; every computation runs in the HIP kernels
+of libhh.so, and this module refuses to run without that library.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhh.so")
+
+HH_OK, HH_ERR_ARG, HH_ERR_DEVICE, HH_ERR_NOMEM, HH_ERR_UNSUPPORTED, HH_ERR_SINGULAR = range(6)
+PREC_F64, PREC_MIXED = 0, 1
+
+_lib = None
+
+
+class HHError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"[{status}] {msg}")
+        self.status = status
+
+
+def lib():
+    """Load libhh.so; raises if the HIP library was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m hackathon_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.hh_last_error.restype = ctypes.c_char_p
+        L.hh_status_string.restype = ctypes.c_char_p
+        L.hh_context_get_stream.restype = ctypes.c_void_p
+        L.hh_context_bytes_in_use.restype = ctypes.c_size_t
+        L.hh_plan_field_size.restype = ctypes.c_long
+        L.hh_plan_workspace_bytes.restype = ctypes.c_size_t
+        for name in ("hh_dst_rows", "hh_dst2d_layers"):
+            getattr(L, name).argtypes = None
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != HH_OK:
+        raise HHError(rc, lib().hh_last_error().decode())
+
+
+def _dptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _hptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c128(a, n=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128))
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} complex values, got {a.size}")
+    return a
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _require_cuda(t, name):
+    if not (t.is_cuda and t.dtype == _torch().complex128 and t.is_contiguous()):
+        raise TypeError(f"{name} must be a contiguous complex128 tensor on the GPU")
+
+
+class Context:
+    """Device, stream, twiddle tables and scratch of the library."""
+
+    def __init__(self, device=0):
+        torch = _torch()
+        self.device = device
+        torch.cuda.set_device(device)
+        h = ctypes.c_void_p()
+        _check(lib().hh_context_create(device, ctypes.byref(h)))
+        self.h = h
+        self.bind_stream()
+
+    def bind_stream(self, stream=None):
+        """Run on `stream` (default: torch's current stream of this device)."""
+        torch = _torch()
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        _check(lib().hh_context_set_stream(self.h, ctypes.c_void_p(s.cuda_stream)))
+
+    def synchronize(self):
+        _check(lib().hh_context_synchronize(self.h))
+
+    def set_memory_budget(self, nbytes):
+        _check(lib().hh_context_set_memory_budget(self.h, ctypes.c_size_t(int(nbytes))))
+
+    def bytes_in_use(self):
+        return int(lib().hh_context_bytes_in_use(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hh_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- transforms (forwardDST / reverseDST, DST of 3D layers) ----
+    def dst_rows(self, x, out=None):
+        """Orthonormal DST-I of every row (last axis) of a complex128 GPU tensor."""
+        _require_cuda(x, "x")
+        out = x.new_empty(x.shape) if out is None else out
+        N = x.shape[-1]
+        self.bind_stream()
+        _check(lib().hh_dst_rows(self.h, ctypes.c_int(N), ctypes.c_long(x.numel() // N), _dptr(x), _dptr(out)))
+        return out
+
+    def dst2d_layers(self, x, out=None):
+        """DST-I along x then y of tensors shaped (..., Ny, Nx)."""
+        _require_cuda(x, "x")
+        out = x.new_empty(x.shape) if out is None else out
+        Ny, Nx = x.shape[-2], x.shape[-1]
+        self.bind_stream()
+        _check(lib().hh_dst2d_layers(self.h, ctypes.c_int(Nx), ctypes.c_int(Ny),
+                                     ctypes.c_long(x.numel() // (Nx * Ny)), _dptr(x), _dptr(out)))
+        return out
+
+    def norms(self, x, y=None):
+        """(max |x|, max |x - y|) over all entries."""
+        _require_cuda(x, "x")
+        out = (ctypes.c_double * 2)()
+        self.bind_stream()
+        _check(lib().hh_norms(self.h, ctypes.c_long(x.numel()), _dptr(x),
+                              _dptr(y) if y is not None else ctypes.c_void_p(0), out))
+        return out[0], out[1]
+
+
+# ------------------------------------------------------------------ coefficients
+def coefficients2d(order, Nx, Ny, hx, hy, k2e, k2y=None, k2yy=None):
+    """The reference's six 2D stencil vectors [bp, ap, b, a, bm, am].
+
+    k2e: k^2 at y0 + j*hy, j = 0..Ny+1; k2y, k2yy: analytic (k^2)_y, (k^2)_yy
+    at the interior rows (sixth order; None = 0)."""
+    k2e = _c128(k2e, Ny + 2)
+    k2y = None if k2y is None else _c128(k2y, Ny)
+    k2yy = None if k2yy is None else _c128(k2yy, Ny)
+    out = [np.zeros(Ny, np.complex128) for _ in range(6)]
+    null = ctypes.c_void_p(0)
+    _check(lib().hh_coefficients2d(order, Nx, Ny, ctypes.c_double(hx), ctypes.c_double(hy), _hptr(k2e),
+                                   _hptr(k2y) if k2y is not None else null, _hptr(k2yy) if k2yy is not None else null,
+                                   *[_hptr(o) for o in out]))
+    return out
+
+
+def coefficients3d(order, Nx, Ny, Nz, hx, hy, hz, k2e, k2d=None):
+    """The reference's twelve 3D vectors [a, b, c, d, ap, bp, cp, dp, am, bm, cm, dm]."""
+    k2e = _c128(k2e, Nz + 2)
+    k2d = None if k2d is None else _c128(k2d, 4)
+    out = [np.zeros(Nz, np.complex128) for _ in range(12)]
+    arr = (ctypes.c_void_p * 12)(*[o.ctypes.data for o in out])
+    _check(lib().hh_coefficients3d(order, Nx, Ny, Nz, ctypes.c_double(hx), ctypes.c_double(hy),
+                                   ctypes.c_double(hz), _hptr(k2e), _hptr(k2d) if k2d is not None else ctypes.c_void_p(0),
+                                   arr))
+    return out
+
+
+# ------------------------------------------------------------------ plans
+class Plan:
+    dim = 0
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.h = None
+
+    @property
+    def field_size(self):
+        return int(lib().hh_plan_field_size(self.h))
+
+    def workspace_bytes(self, batch=1):
+        return int(lib().hh_plan_workspace_bytes(self.h, batch))
+
+    def _batch_of(self, t):
+        n = self.field_size
+        if t.numel() % n:
+            raise ValueError(f"tensor of {t.numel()} values is not a whole number of fields of {n}")
+        return t.numel() // n
+
+    def solve(self, rhs, out=None):
+        """sol = A^{-1} rhs for a batch of right-hand sides (complex128, GPU)."""
+        _require_cuda(rhs, "rhs")
+        out = rhs.new_empty(rhs.shape) if out is None else out
+        _require_cuda(out, "out")
+        self.ctx.bind_stream()
+        _check(lib().hh_solve(self.h, _dptr(rhs), _dptr(out), self._batch_of(rhs)))
+        return out
+
+    def solve_host(self, rhs):
+        rhs = _c128(rhs)
+        sol = np.empty_like(rhs)
+        self.ctx.bind_stream()
+        _check(lib().hh_solve_host(self.h, _hptr(rhs), _hptr(sol), rhs.size // self.field_size))
+        return sol
+
+    def residual(self, sol, rhs, out=None):
+        """res = A sol - rhs (FP64 compact operator)."""
+        _require_cuda(sol, "sol")
+        _require_cuda(rhs, "rhs")
+        out = sol.new_empty(sol.shape) if out is None else out
+        self.ctx.bind_stream()
+        _check(lib().hh_residual(self.h, _dptr(sol), _dptr(rhs), _dptr(out), self._batch_of(sol)))
+        return out
+
+    def set_refine(self, tol=1e-12, max_iter=6):
+        _check(lib().hh_plan_set_refine(self.h, ctypes.c_double(tol), ctypes.c_int(max_iter)))
+
+    def last_refine(self):
+        s, r = ctypes.c_int(), ctypes.c_double()
+        _check(lib().hh_plan_last_refine(self.h, ctypes.byref(s), ctypes.byref(r)))
+        return s.value, r.value
+
+    def close(self):
+        if self.h:
+            lib().hh_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan2D(Plan):
+    """2D operator: coefficient vectors [bp, ap, b, a, bm, am] of length Ny."""
+    dim = 2
+
+    def __init__(self, ctx, Nx, Ny, coef, precision=PREC_F64):
+        super().__init__(ctx)
+        self.Nx, self.Ny = Nx, Ny
+        self.coef = [_c128(c, Ny) for c in coef]
+        h = ctypes.c_void_p()
+        _check(lib().hh_plan2d_create(ctx.h, Nx, Ny, *[_hptr(c) for c in self.coef], precision, ctypes.byref(h)))
+        self.h = h
+        self.precision = precision
+
+    def rhs(self, order, hx, hy, f_e, u_e, k2e, fy_b=None, fxx_b=None, fyy_b=None, k2yb=None, out=None):
+        """Compact right-hand side (rhs.c) from f and u on the extended grid."""
+        _require_cuda(f_e, "f_e")
+        _require_cuda(u_e, "u_e")
+        batch = f_e.numel() // ((self.Nx + 2) * (self.Ny + 2))
+        out = f_e.new_empty((batch, self.Ny, self.Nx)) if out is None else out
+        k2e = _c128(k2e, self.Ny + 2)
+        k2yb = _c128(np.zeros(2) if k2yb is None else k2yb, 2)
+        null = ctypes.c_void_p(0)
+        dp = lambda t: _dptr(t) if t is not None else null
+        self.ctx.bind_stream()
+        _check(lib().hh_rhs2d(self.h, order, ctypes.c_double(hx), ctypes.c_double(hy), _dptr(f_e), _dptr(u_e),
+                              dp(fy_b), dp(fxx_b), dp(fyy_b), _hptr(k2e), _hptr(k2yb), _dptr(out), batch))
+        return out
+
+
+class Plan3D(Plan):
+    """3D operator: the twelve vectors [a, b, c, d, ap, bp, cp, dp, am, bm, cm, dm] of length Nz."""
+    dim = 3
+
+    def __init__(self, ctx, Nx, Ny, Nz, coef, precision=PREC_F64):
+        super().__init__(ctx)
+        self.Nx, self.Ny, self.Nz = Nx, Ny, Nz
+        if len(coef) == 12 * Nz and np.ndim(coef) == 1:
+            coef = np.asarray(coef).reshape(12, Nz)
+        self.coef = [_c128(c, Nz) for c in coef]
+        arr = (ctypes.c_void_p * 12)(*[c.ctypes.data for c in self.coef])
+        h = ctypes.c_void_p()
+        _check(lib().hh_plan3d_create(ctx.h, Nx, Ny, Nz, arr, precision, ctypes.byref(h)))
+        self.h = h
+        self.precision = precision
+
+    def rhs(self, order, hx, hy, hz, f_e, u_e, k2e, fxx_f=None, fyy_f=None, fzz_f=None, k2d=None, out=None):
+        _require_cuda(f_e, "f_e")
+        _require_cuda(u_e, "u_e")
+        batch = f_e.numel() // ((self.Nx + 2) * (self.Ny + 2) * (self.Nz + 2))
+        out = f_e.new_empty((batch, self.Nz, self.Ny, self.Nx)) if out is None else out
+        k2e = _c128(k2e, self.Nz + 2)
+        k2d = _c128(np.zeros(4) if k2d is None else k2d, 4)
+        null = ctypes.c_void_p(0)
+        dp = lambda t: _dptr(t) if t is not None else null
+        self.ctx.bind_stream()
+        _check(lib().hh_rhs3d(self.h, order, ctypes.c_double(hx), ctypes.c_double(hy), ctypes.c_double(hz),
+                              _dptr(f_e), _dptr(u_e), dp(fxx_f), dp(fyy_f), dp(fzz_f), _hptr(k2e), _hptr(k2d),
+                              _dptr(out), batch))
+        return out

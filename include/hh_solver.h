@@ -1,0 +1,164 @@
+/*
+ * hh_solver.h -- C ABI of the MI355X compact-Helmholtz direct solver.
+ *
+ * Plain pointers and sizes only.  Complex numbers are interleaved
+ * (re, im) doubles: the bytes of C99 `double _Complex`, fftw_complex and
+ * hipDoubleComplex.  Fields are row-major, index i + j*Nx (+ l*Nx*Ny), the
+ * layout of the reference's System arrays (2D_Program/headers/structs.h:9-30,
+ * 3D_Program/headers/structs.h:11-33).  A batch of right-hand sides is
+ * stored back to back (stride Nx*Ny or Nx*Ny*Nz).
+ *
+ * Every function returns HH_OK (0) or a status code; hh_last_error()
+ * gives the message of the last failure on the calling thread.  Functions
+ * taking `d_` pointers expect device memory on the context's device and run
+ * asynchronously on the context's stream; `h_` pointers are host memory and
+ * the call returns when the result is in host memory.
+ */
+#ifndef HH_SOLVER_H
+#define HH_SOLVER_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    HH_OK = 0,
+    HH_ERR_ARG = 1,          /* invalid size, order, pointer or batch */
+    HH_ERR_DEVICE = 2,       /* HIP runtime failure */
+    HH_ERR_NOMEM = 3,        /* device allocation failed or over budget */
+    HH_ERR_UNSUPPORTED = 4,  /* shape outside the kernels' limits */
+    HH_ERR_SINGULAR = 5      /* a mode's pivot vanished (resonance) */
+};
+
+enum { HH_PREC_F64 = 0, HH_PREC_MIXED = 1 };
+
+typedef struct hh_context hh_context;
+typedef struct hh_plan hh_plan;
+
+const char *hh_last_error(void);
+const char *hh_status_string(int status);
+
+/* ---- context: device, stream, twiddle/DST tables, scratch ---- */
+int hh_context_create(int device, hh_context **ctx);
+int hh_context_destroy(hh_context *ctx);
+/* run on a caller stream (hipStream_t); NULL restores the context's own */
+int hh_context_set_stream(hh_context *ctx, void *stream);
+void *hh_context_get_stream(hh_context *ctx);
+int hh_context_synchronize(hh_context *ctx);
+/* limit on device bytes the context may allocate (0 = no limit) */
+int hh_context_set_memory_budget(hh_context *ctx, size_t bytes);
+size_t hh_context_bytes_in_use(const hh_context *ctx);
+/* Kernel timing with HIP events recorded on the launch stream around each
+ * pass.  Categories: 0 = DST (FFT), 1 = DST (dense), 2 = tridiagonal,
+ * 3 = residual / refinement update, 4 = right-hand side.  read_timing
+ * synchronises the stream and returns, per category, the summed kernel
+ * milliseconds, the number of launches and the algorithmic bytes moved
+ * (each element read once and written once). */
+int hh_context_enable_timing(hh_context *ctx, int on);
+int hh_context_read_timing(hh_context *ctx, double ms[5], long launches[5], double bytes[5]);
+int hh_context_reset_timing(hh_context *ctx);
+
+/* ---- stencil coefficients (host, O(N)) ----
+ * 2D: replaces coefficients() of 2D_Program/main_functions/coefficients.c:
+ * 9-67.  k2e: k^2 at y0 + j*hy, j = 0..Ny+1 (complex, 2*(Ny+2) doubles);
+ * k2y, k2yy: analytic (k^2)_y and (k^2)_yy at the interior rows (complex,
+ * 2*Ny doubles, NULL = zero; sixth order only).  Outputs: the reference's
+ * bp, ap, b, a, bm, am (each complex, length Ny). */
+int hh_coefficients2d(int order, int Nx, int Ny, double hx, double hy, const double *k2e,
+                      const double *k2y, const double *k2yy, double *bp, double *ap, double *b,
+                      double *a, double *bm, double *am);
+/* 3D: replaces coefficients() of 3D_Program/main_functions/coefficients.c:
+ * 7-111 (with derivative_kz/kzz_4th_order of 4th_order_partials.c:296-385).
+ * k2e: k^2 at z0 + l*hz, l = 0..Nz+1; k2d: analytic (k^2)_z(z0), (k^2)_z(z1),
+ * (k^2)_zz(z0), (k^2)_zz(z1) (complex, 8 doubles; NULL = zero).
+ * coef[12]: a, b, c, d, ap, bp, cp, dp, am, bm, cm, dm (each length Nz). */
+int hh_coefficients3d(int order, int Nx, int Ny, int Nz, double hx, double hy, double hz,
+                      const double *k2e, const double *k2d, double *const coef[12]);
+
+/* ---- plans: one operator, device-resident coefficients and eigenvalues ----
+ * Replace the precomputation of LU() (2D_Program/main_functions/LU.c:3-46,
+ * 3D_Program/main_functions/LU.c:4-81): no N^2/N^3 factors are stored, the
+ * per-mode tridiagonal bands are rebuilt on the fly from these vectors. */
+int hh_plan2d_create(hh_context *ctx, int Nx, int Ny, const double *bp, const double *ap,
+                     const double *b, const double *a, const double *bm, const double *am,
+                     int precision, hh_plan **plan);
+int hh_plan3d_create(hh_context *ctx, int Nx, int Ny, int Nz, const double *const coef[12],
+                     int precision, hh_plan **plan);
+int hh_plan_destroy(hh_plan *plan);
+hh_context *hh_plan_context(const hh_plan *plan);
+/* grid sizes of a plan: dims[0..2] = Nx, Ny, Nz (Nz = 1 in 2D); returns 2 or 3 */
+int hh_plan_dims(const hh_plan *plan, int dims[3]);
+/* elements of one right-hand side */
+long hh_plan_field_size(const hh_plan *plan);
+/* device bytes one solve of `batch` right-hand sides needs beyond its
+ * input and output (scratch for non power-of-two sizes, refinement) */
+size_t hh_plan_workspace_bytes(const hh_plan *plan, int batch);
+
+/* ---- solve: replaces solver(System) ----
+ * 2D_Program/solver_functions/solver.c:14-117 and
+ * 3D_Program/solver_functions/solver.c:55-249 / :263-413.
+ * sol = A^{-1} rhs for `batch` right-hand sides; d_sol may equal d_rhs.
+ * HH_PREC_MIXED plans solve in FP32 and refine with the FP64 compact
+ * residual until ||r||/||rhs|| <= tol or max_iter sweeps (hh_plan_set_refine). */
+int hh_solve(hh_plan *plan, const void *d_rhs, void *d_sol, int batch);
+int hh_solve_host(hh_plan *plan, const double *h_rhs, double *h_sol, int batch);
+int hh_plan_set_refine(hh_plan *plan, double tol, int max_iter);
+/* sweeps and final relative residual of the last mixed solve */
+int hh_plan_last_refine(const hh_plan *plan, int *sweeps, double *rel_residual);
+
+/* ---- residual: replaces residual(System) ----
+ * 2D_Program/main_functions/residual.c:3-34 (9-point) and
+ * 3D_Program/main_functions/residual.c:42-96 (27-point): res = A sol - rhs. */
+int hh_residual(hh_plan *plan, const void *d_sol, const void *d_rhs, void *d_res, int batch);
+
+/* ---- transforms ----
+ * hh_dst_rows replaces forwardDST/reverseDST (2D_Program/solver_functions/
+ * DST.c:6-44): orthonormal DST-I of nrows contiguous rows of length N.
+ * hh_dst2d_layers replaces DST()/DST1()/DST2() (3D_Program/solver_functions/
+ * DST.c:13-204): DST-I along x then y of nlayers Nx*Ny layers.
+ * Out of place or in place. */
+int hh_dst_rows(hh_context *ctx, int N, long nrows, const void *d_in, void *d_out);
+int hh_dst2d_layers(hh_context *ctx, int Nx, int Ny, long nlayers, const void *d_in, void *d_out);
+
+/* ---- compact right-hand side: replaces rhs(System) ----
+ * 2D_Program/main_functions/rhs.c:4-150 + partial_derivatives (all files).
+ * Per right-hand side (device, complex): f_e = f on the extended
+ * (Nx+2)(Ny+2) grid; u_e = Dirichlet data on the extended grid; fy_b =
+ * f_y at (x_i, y0) then (x_i, y1) (2*Nx); fxx_b, fyy_b = f_xx, f_yy on the
+ * ghost ring: bottom row, top row, left column, right column
+ * (2(Nx+2)+2(Ny+2)).  Host, complex: k2e = k^2 at y0 + j*hy (Ny+2), k2yb =
+ * (k^2)_y at y0, y1 (2).  Orders 2 and 4 ignore fy_b, fxx_b, fyy_b, k2yb. */
+int hh_rhs2d(hh_plan *plan, int order, double hx, double hy, const void *d_f_e, const void *d_u_e,
+             const void *d_fy_b, const void *d_fxx_b, const void *d_fyy_b, const double *k2e,
+             const double *k2yb, void *d_F, int batch);
+/* 3D_Program/main_functions/rhs.c:13-294 + partial_derivatives.  f_e =
+ * hz^2 f on the extended grid, u_e = u on the extended grid (z faces used);
+ * fxx_f = hz^2 f_xx on the x faces [face][l][j] (2(Ny+2)(Nz+2)), fyy_f on
+ * the y faces [face][l][i], fzz_f on the z faces [face][j][i]; host: k2e =
+ * k^2 at z0 + l*hz (Nz+2), k2d as in hh_coefficients3d. */
+int hh_rhs3d(hh_plan *plan, int order, double hx, double hy, double hz, const void *d_f_e,
+             const void *d_u_e, const void *d_fxx_f, const void *d_fyy_f, const void *d_fzz_f,
+             const double *k2e, const double *k2d, void *d_F, int batch);
+
+/* ---- host-pointer variants (copy in, run, copy out, synchronise) ---- */
+int hh_residual_host(hh_plan *plan, const double *h_sol, const double *h_rhs, double *h_res, int batch);
+int hh_dst_rows_host(hh_context *ctx, int N, long nrows, const double *h_in, double *h_out);
+int hh_dst2d_layers_host(hh_context *ctx, int Nx, int Ny, long nlayers, const double *h_in, double *h_out);
+/* sizes: f_e, u_e (Nx+2)(Ny+2); fy_b 2Nx; rings 2(Nx+2)+2(Ny+2); F Nx*Ny */
+int hh_rhs2d_host(hh_plan *plan, int order, double hx, double hy, const double *f_e, const double *u_e,
+                  const double *fy_b, const double *fxx_b, const double *fyy_b, const double *k2e,
+                  const double *k2yb, double *F);
+int hh_rhs3d_host(hh_plan *plan, int order, double hx, double hy, double hz, const double *f_e,
+                  const double *u_e, const double *fxx_f, const double *fyy_f, const double *fzz_f,
+                  const double *k2e, const double *k2d, double *F);
+
+/* ---- norms: normInf of 2D_Program/helpers_functions/norms.c:3-13 ----
+ * out[0] = max |x|, out[1] = max |x - y| (0 when d_y is NULL); blocks. */
+int hh_norms(hh_context *ctx, long n, const void *d_x, const void *d_y, double out[2]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,237 @@
+"""GPU parity: every kernel set against the CPU FP64 oracle on the same inputs.
+
+Tolerances (relative, max norm): FP64 paths <= 1e-12 against the oracle
+(north-star bar); the DST alone <= 1e-13.  Mixed precision <= 1e-10.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def H():
+    import torch
+    assert torch.cuda.is_available(), "GPU test run without a GPU"
+    from hackathon_amd import helmholtz
+    return helmholtz
+
+
+@pytest.fixture(scope="module")
+def ctx(H):
+    return H.Context(0)
+
+
+def dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.complex128)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def relerr(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+# ------------------------------------------------------------------ DST
+def test_dst_golden(ctx):
+    g = np.load(os.path.join(GOLD, "dst1_golden.npz"))
+    for key in g.files:
+        if not key.startswith("x_"):
+            continue
+        n = key[2:]
+        say(f"dst golden N={n}")
+        y = host(ctx.dst_rows(dev(g[key])))
+        assert relerr(y, g["y_" + n]) <= 1e-13
+
+
+@pytest.mark.parametrize("N,rows", [(1, 5), (3, 7), (7, 300), (15, 33), (31, 129), (63, 64), (100, 37), (101, 3),
+                                    (127, 1000), (200, 9), (255, 513), (511, 65), (1023, 257), (2047, 17),
+                                    (4095, 5)])
+def test_dst_rows_vs_oracle(ctx, oracle, N, rows):
+    say(f"dst rows N={N} rows={rows}")
+    rng = np.random.default_rng(N * 7 + rows)
+    x = rng.standard_normal((rows, N)) + 1j * rng.standard_normal((rows, N))
+    ref = oracle.dst1_rows(x)
+    y = host(ctx.dst_rows(dev(x)))
+    assert relerr(y, ref) <= 1e-13
+    # in place, and self-inverse
+    t = dev(x)
+    ctx.dst_rows(t, out=t)
+    ctx.dst_rows(t, out=t)
+    assert relerr(host(t), x) <= 1e-13
+
+
+@pytest.mark.parametrize("Nx,Ny,nl", [(15, 15, 3), (31, 63, 2), (63, 31, 4), (255, 255, 2), (20, 15, 3), (15, 24, 2),
+                                      (127, 7, 5)])
+def test_dst2d_layers(ctx, oracle, Nx, Ny, nl):
+    say(f"dst2d layers {Nx}x{Ny}x{nl}")
+    rng = np.random.default_rng(Nx + 100 * Ny)
+    x = rng.standard_normal((nl, Ny, Nx)) + 1j * rng.standard_normal((nl, Ny, Nx))
+    ref = oracle.dst1_rows(x)                                       # along x
+    ref = np.swapaxes(oracle.dst1_rows(np.swapaxes(ref, 1, 2).copy()), 1, 2)   # along y
+    y = host(ctx.dst2d_layers(dev(x)))
+    assert relerr(y, ref) <= 1e-13
+
+
+# ------------------------------------------------------------------ 2D solve
+CASES2D = [
+    (2, 0, None, 31), (4, 0, None, 63), (6, 0, None, 127), (6, 0, None, 100),
+    (4, 1, [10.0, 0.0, 3, 2, 0.0], 255), (6, 1, [10.0, 0.0, 3, 2, 0.0], 100),
+    (6, 1, [50.0, 0.0, 7, 11, 1.3], 1023), (4, 1, [50.0, 0.0, 20, 1, 4.0], 1023),
+    (6, 1, [80.0, 0.5, 5, 9, 0.2], 2047), (4, 1, [30.0, 0.0, 2, 3, 0.0], 2),
+]
+
+
+def _ref2d(oracle, order, kind, p, N):
+    return oracle.solve2d(order, N, kind, p, threads=8)
+
+
+@pytest.mark.parametrize("order,kind,p,N", CASES2D)
+def test_solve2d_vs_oracle(H, ctx, oracle, order, kind, p, N):
+    say(f"solve2d order={order} kind={kind} N={N}")
+    s = _ref2d(oracle, order, kind, p, N)
+    plan = H.Plan2D(ctx, N, N, s["coef"])
+    sol = host(plan.solve(dev(s["rhs"])))
+    assert relerr(sol, s["sol"]) <= 1e-12
+    # the same through the host entry point
+    assert relerr(plan.solve_host(s["rhs"]), s["sol"]) <= 1e-12
+
+
+def test_solve2d_rectangular(H, ctx, oracle):
+    say("solve2d 63x40 order 4")
+    s = oracle.solve2d(4, 63, 0, None, Ny=40)
+    plan = H.Plan2D(ctx, 63, 40, s["coef"])
+    assert relerr(host(plan.solve(dev(s["rhs"]))), s["sol"]) <= 1e-12
+
+
+@pytest.mark.parametrize("N", [63, 1023])
+def test_batch_alone_equals_in_batch(H, ctx, oracle, N):
+    """Each right-hand side gives the same bits alone and inside a batch."""
+    say(f"batch determinism N={N}")
+    s = oracle.setup2d(6, N, N, 1, [50.0, 0.0, 3, 4, 0.5])
+    plan = H.Plan2D(ctx, N, N, s["coef"])
+    rng = np.random.default_rng(5)
+    B = 5
+    rhs = rng.standard_normal((B, N * N)) + 1j * rng.standard_normal((B, N * N))
+    allsol = host(plan.solve(dev(rhs)))
+    for b in (0, 3):
+        alone = host(plan.solve(dev(rhs[b])))
+        assert np.array_equal(alone, allsol[b])
+
+
+# ------------------------------------------------------------------ 3D solve
+CASES3D = [
+    (2, 0, [10, 9, 10, 9], 15), (4, 0, [10, 9, 10, 9], 31), (6, 0, [10, 9, 10, 9], 31),
+    (6, 0, [10, 9, 10, 9], 20), (6, 1, [30.0, 0.0, 3, 2, 1, 1.0, 1.0], 63),
+    (4, 1, [30.0, 0.0, 5, 1, 2, 1.0, 1.0], 63), (6, 1, [30.0, 0.0, 3, 2, 1, 1.0, 1.0], 24),
+]
+
+
+@pytest.mark.parametrize("order,kind,p,N", CASES3D)
+def test_solve3d_vs_oracle(H, ctx, oracle, order, kind, p, N):
+    say(f"solve3d order={order} kind={kind} N={N}")
+    s = oracle.solve3d(order, N, kind, p, threads=8)
+    plan = H.Plan3D(ctx, N, N, N, s["coef"])
+    sol = host(plan.solve(dev(s["rhs"])))
+    assert relerr(sol, s["sol"]) <= 1e-12
+
+
+def test_solve3d_anisotropic(H, ctx, oracle):
+    say("solve3d 31x15x20 order 4")
+    s = oracle.solve3d(4, 31, 0, [10, 9, 10, 9], Ny=15, Nz=20)
+    plan = H.Plan3D(ctx, 31, 15, 20, s["coef"])
+    assert relerr(host(plan.solve(dev(s["rhs"]))), s["sol"]) <= 1e-12
+
+
+# ------------------------------------------------------------------ residual
+@pytest.mark.parametrize("order,N", [(4, 63), (6, 100)])
+def test_residual2d(H, ctx, oracle, order, N):
+    say(f"residual2d order={order} N={N}")
+    s = oracle.solve2d(order, N, 0, None)
+    plan = H.Plan2D(ctx, N, N, s["coef"])
+    rng = np.random.default_rng(1)
+    u = rng.standard_normal(N * N) + 1j * rng.standard_normal(N * N)
+    ref = oracle.residual2d(N, N, s["coef"], u, s["rhs"])
+    r = host(plan.residual(dev(u), dev(s["rhs"])))
+    assert relerr(r, ref) <= 1e-13
+
+
+@pytest.mark.parametrize("order,N", [(4, 15), (6, 20)])
+def test_residual3d(H, ctx, oracle, order, N):
+    say(f"residual3d order={order} N={N}")
+    s = oracle.setup3d(order, N, N, N, 0, [10, 9, 10, 9])
+    plan = H.Plan3D(ctx, N, N, N, s["coef"])
+    rng = np.random.default_rng(2)
+    u = rng.standard_normal(N ** 3) + 1j * rng.standard_normal(N ** 3)
+    ref = oracle.residual3d(N, N, N, s["coef"], u, s["rhs"])
+    r = host(plan.residual(dev(u), dev(s["rhs"])))
+    assert relerr(r, ref) <= 1e-13
+
+
+# ------------------------------------------------------------------ RHS assembly
+@pytest.mark.parametrize("order,kind,p,N", [(2, 0, None, 31), (4, 0, None, 40), (6, 0, None, 63),
+                                            (6, 1, [50.0, 0.0, 7, 11, 1.3], 127), (6, 0, None, 2)])
+def test_rhs2d(H, ctx, oracle, order, kind, p, N):
+    say(f"rhs2d order={order} kind={kind} N={N}")
+    s = oracle.setup2d(order, N, N, kind, p)
+    fl = oracle.fields2d(N, N, kind, p)
+    dom = oracle.domain2d(kind, p)
+    hx, hy = (dom[1] - dom[0]) / (N + 1), (dom[3] - dom[2]) / (N + 1)
+    plan = H.Plan2D(ctx, N, N, s["coef"])
+    F = plan.rhs(order, hx, hy, dev(fl["f_e"]), dev(fl["u_e"]), fl["k2_e"], fy_b=dev(fl["fy_b"]),
+                 fxx_b=dev(fl["fxx_b"]), fyy_b=dev(fl["fyy_b"]), k2yb=fl["k2y_b"])
+    assert relerr(host(F).ravel(), s["rhs"]) <= 1e-12
+
+
+@pytest.mark.parametrize("order,kind,p,N", [(2, 0, [10, 9, 10, 9], 15), (4, 0, [10, 9, 10, 9], 20),
+                                            (6, 0, [10, 9, 10, 9], 31), (6, 1, [30.0, 0.0, 3, 2, 1, 1.0, 1.0], 24)])
+def test_rhs3d(H, ctx, oracle, order, kind, p, N):
+    say(f"rhs3d order={order} kind={kind} N={N}")
+    s = oracle.setup3d(order, N, N, N, kind, p)
+    fl = oracle.fields3d(N, N, N, kind, p)
+    dom = oracle.domain3d(kind, p)
+    h = [(dom[2 * a + 1] - dom[2 * a]) / (N + 1) for a in range(3)]
+    plan = H.Plan3D(ctx, N, N, N, s["coef"])
+    F = plan.rhs(order, *h, dev(fl["f_e"]), dev(fl["u_e"]), fl["k2_e"], fxx_f=dev(fl["fxx_f"]),
+                 fyy_f=dev(fl["fyy_f"]), fzz_f=dev(fl["fzz_f"]), k2d=fl["k2z_b"])
+    assert relerr(host(F).ravel(), s["rhs"]) <= 1e-12
+
+
+# ------------------------------------------------------------------ mixed precision
+@pytest.mark.parametrize("dim,order,N", [(2, 6, 255), (2, 4, 1023), (3, 6, 63)])
+def test_mixed_precision(H, ctx, oracle, dim, order, N):
+    say(f"mixed dim={dim} order={order} N={N}")
+    if dim == 2:
+        s = oracle.solve2d(order, N, 1, [50.0, 0.0, 3, 7, 0.4], threads=8)
+        plan = H.Plan2D(ctx, N, N, s["coef"], precision=H.PREC_MIXED)
+    else:
+        s = oracle.solve3d(order, N, 1, [30.0, 0.0, 3, 2, 1, 1.0, 1.0], threads=8)
+        plan = H.Plan3D(ctx, N, N, N, s["coef"], precision=H.PREC_MIXED)
+    plan.set_refine(1e-13, 8)
+    sol = host(plan.solve(dev(s["rhs"])))
+    sweeps, rel = plan.last_refine()
+    say(f"  sweeps={sweeps} relres={rel:.2e} err={relerr(sol, s['sol']):.2e}")
+    assert relerr(sol, s["sol"]) <= 1e-10
+
+
+# ------------------------------------------------------------------ errors
+def test_bad_arguments_raise(H, ctx):
+    with pytest.raises(H.HHError):
+        H.Plan2D(ctx, 1, 1, [np.zeros(1)] * 6)
+    plan = H.Plan2D(ctx, 7, 7, [np.ones(7)] * 6)
+    import torch
+    with pytest.raises(ValueError):
+        plan.solve(torch.zeros(50, dtype=torch.complex128, device="cuda"))
