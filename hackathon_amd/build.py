@@ -65,6 +65,16 @@ def build(verbose=False):
         _run(["gcc", "-O2", "-fPIC", "-shared", "-std=gnu99", "-Wall", f"-I{INCLUDE}", "-o", tmp, path,
               f"-L{HERE}", "-lhh", "-Wl,-rpath,$ORIGIN", "-lm"])
         os.replace(tmp, out)
+    # Direct_Solver drivers (order N [threads]); -rdynamic exports their
+    # user-input functions to the drop-in libraries
+    for src, lib, out in (("hh_driver2d.c", "hh_ref2d", "Direct_Solver2d"), ("hh_driver3d.c", "hh_ref3d", "Direct_Solver3d")):
+        path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            continue
+        exe = os.path.join(HERE, out)
+        _run(["gcc", "-O2", "-std=gnu99", "-Wall", "-rdynamic", f"-I{INCLUDE}", "-o", exe + ".tmp", path,
+              f"-L{HERE}", f"-l{lib}", "-lhh", "-Wl,-rpath,$ORIGIN", "-lm"])
+        os.replace(exe + ".tmp", exe)
     if verbose:
         print("built", LIB)
     return LIB

@@ -91,7 +91,8 @@ struct TimeScope {
 struct hh_plan {
     hh_context *ctx = nullptr;
     int dim = 2, Nx = 0, Ny = 0, Nz = 1, prec = HH_PREC_F64;
-    void *coef64 = nullptr, *coef32 = nullptr;   // [q][6] or [q][12]
+    int nop = 1;                                 // operators (one per right-hand side b % nop)
+    void *coef64 = nullptr, *coef32 = nullptr;   // [op][q][6] or [op][q][12]
     double *lamx = nullptr, *lamy = nullptr;
     std::vector<cd> hcoef;                       // host copy of the band table
     double tol = 1e-12;
@@ -282,7 +283,9 @@ extern "C" int hh_context_destroy(hh_context *c) {
 
 extern "C" int hh_context_set_stream(hh_context *c, void *stream) {
     if (!c) return fail(HH_ERR_ARG, "null context");
-    c->stream = stream ? (hipStream_t)stream : c->own;
+    // NULL is the HIP default (null) stream, which orders against the work
+    // of callers (e.g. PyTorch's default stream) that use it
+    c->stream = (hipStream_t)stream;
     return HH_OK;
 }
 
@@ -462,14 +465,14 @@ extern "C" int hh_coefficients3d(int order, int Nx, int Ny, int Nz, double hx, d
 // ------------------------------------------------------------------ plans
 static int upload_plan(hh_plan *p, int ncoef, int n) {
     hh_context *c = p->ctx;
-    size_t b64 = (size_t)ncoef * n * 16, b32 = (size_t)ncoef * n * 8;
+    size_t b64 = (size_t)ncoef * n * p->nop * 16, b32 = (size_t)ncoef * n * p->nop * 8;
     int rc = dev_alloc(c, &p->coef64, b64);
     if (rc) return rc;
     rc = dev_alloc(c, &p->coef32, b32);
     if (rc) return rc;
     HIPCHK(hipMemcpy(p->coef64, p->hcoef.data(), b64, hipMemcpyHostToDevice));
-    std::vector<float> h32(2 * (size_t)ncoef * n);
-    for (size_t i = 0; i < (size_t)ncoef * n; i++) {
+    std::vector<float> h32(2 * (size_t)ncoef * n * p->nop);
+    for (size_t i = 0; i < (size_t)ncoef * n * p->nop; i++) {
         h32[2 * i] = (float)p->hcoef[i].real();
         h32[2 * i + 1] = (float)p->hcoef[i].imag();
     }
@@ -506,18 +509,30 @@ extern "C" int hh_plan2d_create(hh_context *ctx, int Nx, int Ny, const double *b
                                 const double *b, const double *a, const double *bm, const double *am, int precision,
                                 hh_plan **out) {
     if (!ctx || !out || !bp || !ap || !b || !a || !bm || !am) return fail(HH_ERR_ARG, "hh_plan2d_create: null argument");
+    if (Ny < 1) return fail(HH_ERR_ARG, "grid sizes must be at least 2 (got %d x %d)", Nx, Ny);
+    const double *v[6] = {bp, ap, b, a, bm, am};
+    std::vector<double> packed(2 * 6 * (size_t)Ny);
+    for (int q = 0; q < 6; q++) memcpy(packed.data() + 2 * (size_t)q * Ny, v[q], 16 * (size_t)Ny);
+    return hh_plan2d_create_multi(ctx, Nx, Ny, 1, packed.data(), precision, out);
+}
+
+extern "C" int hh_plan2d_create_multi(hh_context *ctx, int Nx, int Ny, int nop, const double *coef, int precision,
+                                      hh_plan **out) {
+    if (!ctx || !out || !coef) return fail(HH_ERR_ARG, "hh_plan2d_create_multi: null argument");
+    if (nop < 1) return fail(HH_ERR_ARG, "number of operators must be >= 1 (got %d)", nop);
     if (precision != HH_PREC_F64 && precision != HH_PREC_MIXED) return fail(HH_ERR_ARG, "unknown precision %d", precision);
     int rc = plan_check_dims(2, Nx, Ny, 1);
     if (rc) return rc;
     if ((rc = set_device(ctx))) return rc;
     hh_plan *p = new hh_plan();
-    p->ctx = ctx; p->dim = 2; p->Nx = Nx; p->Ny = Ny; p->Nz = 1; p->prec = precision;
-    p->hcoef.resize(6 * (size_t)Ny);
-    for (int q = 0; q < Ny; q++) {   // device band table order: bm, am, b, a, bp, ap
-        p->hcoef[6 * q + 0] = ld(bm, q); p->hcoef[6 * q + 1] = ld(am, q);
-        p->hcoef[6 * q + 2] = ld(b, q);  p->hcoef[6 * q + 3] = ld(a, q);
-        p->hcoef[6 * q + 4] = ld(bp, q); p->hcoef[6 * q + 5] = ld(ap, q);
-    }
+    p->ctx = ctx; p->dim = 2; p->Nx = Nx; p->Ny = Ny; p->Nz = 1; p->prec = precision; p->nop = nop;
+    p->hcoef.resize(6 * (size_t)Ny * nop);
+    // input [op][bp, ap, b, a, bm, am][Ny]; device band table [op][q](bm, am, b, a, bp, ap)
+    static const int src[6] = {4, 5, 2, 3, 0, 1};
+    for (int o = 0; o < nop; o++)
+        for (int q = 0; q < Ny; q++)
+            for (int t = 0; t < 6; t++)
+                p->hcoef[((size_t)o * Ny + q) * 6 + t] = ld(coef, ((size_t)o * 6 + src[t]) * Ny + q);
     if ((rc = upload_plan(p, 6, Ny))) { hh_plan_destroy(p); return rc; }
     *out = p;
     return HH_OK;
@@ -528,27 +543,42 @@ extern "C" int hh_plan3d_create(hh_context *ctx, int Nx, int Ny, int Nz, const d
     if (!ctx || !out || !coef) return fail(HH_ERR_ARG, "hh_plan3d_create: null argument");
     for (int q = 0; q < 12; q++)
         if (!coef[q]) return fail(HH_ERR_ARG, "hh_plan3d_create: null coefficient array %d", q);
+    if (Nz < 1) return fail(HH_ERR_ARG, "grid sizes must be at least 2 (got Nz = %d)", Nz);
+    std::vector<double> packed(2 * 12 * (size_t)Nz);
+    for (int q = 0; q < 12; q++) memcpy(packed.data() + 2 * (size_t)q * Nz, coef[q], 16 * (size_t)Nz);
+    return hh_plan3d_create_multi(ctx, Nx, Ny, Nz, 1, packed.data(), precision, out);
+}
+
+extern "C" int hh_plan3d_create_multi(hh_context *ctx, int Nx, int Ny, int Nz, int nop, const double *coef,
+                                      int precision, hh_plan **out) {
+    if (!ctx || !out || !coef) return fail(HH_ERR_ARG, "hh_plan3d_create_multi: null argument");
+    if (nop < 1) return fail(HH_ERR_ARG, "number of operators must be >= 1 (got %d)", nop);
     if (precision != HH_PREC_F64 && precision != HH_PREC_MIXED) return fail(HH_ERR_ARG, "unknown precision %d", precision);
     int rc = plan_check_dims(3, Nx, Ny, Nz);
     if (rc) return rc;
     if ((rc = set_device(ctx))) return rc;
     hh_plan *p = new hh_plan();
-    p->ctx = ctx; p->dim = 3; p->Nx = Nx; p->Ny = Ny; p->Nz = Nz; p->prec = precision;
-    p->hcoef.resize(12 * (size_t)Nz);
-    for (int l = 0; l < Nz; l++)
-        for (int q = 0; q < 12; q++) p->hcoef[12 * l + q] = ld(coef[q], l);
+    p->ctx = ctx; p->dim = 3; p->Nx = Nx; p->Ny = Ny; p->Nz = Nz; p->prec = precision; p->nop = nop;
+    p->hcoef.resize(12 * (size_t)Nz * nop);
+    // input [op][a..dm][Nz]; device table [op][l][12]
+    for (int o = 0; o < nop; o++)
+        for (int l = 0; l < Nz; l++)
+            for (int q = 0; q < 12; q++)
+                p->hcoef[((size_t)o * Nz + l) * 12 + q] = ld(coef, ((size_t)o * 12 + q) * Nz + l);
     if ((rc = upload_plan(p, 12, Nz))) { hh_plan_destroy(p); return rc; }
     *out = p;
     return HH_OK;
 }
+
+extern "C" int hh_plan_num_operators(const hh_plan *p) { return p ? p->nop : 0; }
 
 extern "C" int hh_plan_destroy(hh_plan *p) {
     if (!p) return HH_OK;
     hh_context *c = p->ctx;
     int nc = p->dim == 2 ? 6 : 12, n = p->dim == 2 ? p->Ny : p->Nz;
     (void)hipStreamSynchronize(c->stream);
-    dev_free(c, p->coef64, (size_t)nc * n * 16);
-    dev_free(c, p->coef32, (size_t)nc * n * 8);
+    dev_free(c, p->coef64, (size_t)nc * n * p->nop * 16);
+    dev_free(c, p->coef32, (size_t)nc * n * p->nop * 8);
     dev_free(c, p->lamx, (size_t)p->Nx * 8);
     if (p->lamy) dev_free(c, p->lamy, (size_t)p->Ny * 8);
     delete p;
@@ -604,6 +634,8 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
     t.field = hh_plan_field_size(p);
     t.batch = batch;
     t.coef = tri_comp == 0 ? p->coef64 : p->coef32;
+    t.nop = p->nop;
+    t.cstride = p->dim == 2 ? 6L * Ny : 12L * Nz;
     t.lamx = p->lamx; t.lamy = p->lamy;
     int n = p->dim == 2 ? Ny : Nz;
     long nmodes = p->dim == 2 ? Nx : (long)Nx * Ny;
@@ -655,8 +687,10 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
 static int residual_into(hh_plan *p, const void *u, const void *F, void *res, int res_f32, int batch) {
     TimeScope ts(p->ctx, 3, (double)hh_plan_field_size(p) * batch * (32 + (res_f32 ? 8 : 16)));
     hipError_t e;
-    if (p->dim == 2) e = hh::launch_residual2d(p->Nx, p->Ny, batch, p->coef64, u, F, res, res_f32, p->ctx->stream);
-    else e = hh::launch_residual3d(p->Nx, p->Ny, p->Nz, batch, p->coef64, u, F, res, res_f32, p->ctx->stream);
+    if (p->dim == 2)
+        e = hh::launch_residual2d(p->Nx, p->Ny, batch, p->coef64, p->nop, u, F, res, res_f32, p->ctx->stream);
+    else
+        e = hh::launch_residual3d(p->Nx, p->Ny, p->Nz, batch, p->coef64, p->nop, u, F, res, res_f32, p->ctx->stream);
     if (e != hipSuccess) return fail(HH_ERR_DEVICE, "residual launch failed: %s", hipGetErrorString(e));
     return HH_OK;
 }
@@ -804,21 +838,26 @@ extern "C" int hh_rhs2d(hh_plan *p, int order, double hx, double hy, const void 
     const long XY = (long)(Nx + 2) * (Ny + 2), wstride = 2 * XY + (long)Nx * Ny;
     // per-row k^2 and compact (k^2)_y, uploaded after the field work area
     size_t wbytes = order == 6 ? (size_t)wstride * batch * 16 : 0;
-    size_t tbytes = (size_t)2 * Ny * 16;
+    const int nop = p->nop;
+    size_t tbytes = (size_t)2 * Ny * nop * 16;
     if ((rc = ensure_scratch(c, wbytes + tbytes))) return rc;
-    std::vector<cd> tab(2 * (size_t)Ny);
+    // tables [op][j]: k^2 at the interior rows, then compact (k^2)_y
+    std::vector<cd> tab(2 * (size_t)Ny * nop);
     std::vector<cd> k2y;
-    if (order == 6) k2_y_compact(Ny, hy, k2e, k2yb, k2y);
-    for (int j = 0; j < Ny; j++) {
-        tab[j] = ld(k2e, j + 1);
-        tab[Ny + j] = order == 6 ? k2y[j] : cd(0);
+    for (int o = 0; o < nop; o++) {
+        const double *ke = k2e + 2 * (size_t)o * (Ny + 2);
+        if (order == 6) k2_y_compact(Ny, hy, ke, k2yb ? k2yb + 4 * (size_t)o : nullptr, k2y);
+        for (int j = 0; j < Ny; j++) {
+            tab[(size_t)o * Ny + j] = ld(ke, j + 1);
+            tab[(size_t)(nop + o) * Ny + j] = order == 6 ? k2y[j] : cd(0);
+        }
     }
     char *tb = (char *)c->scratch + wbytes;
     HIPCHK(hipMemcpyAsync(tb, tab.data(), tbytes, hipMemcpyHostToDevice, c->stream));
     hh::Rhs2D r;
     r.order = order; r.Nx = Nx; r.Ny = Ny; r.batch = batch; r.hx = hx; r.hy = hy;
     r.f_e = f_e; r.u_e = u_e; r.fy_b = fy_b; r.fxx_b = fxx_b; r.fyy_b = fyy_b;
-    r.k2c = tb; r.k2y = tb + (size_t)Ny * 16; r.coef = p->coef64; r.work = c->scratch;
+    r.k2c = tb; r.k2y = tb + (size_t)Ny * nop * 16; r.coef = p->coef64; r.nop = nop; r.work = c->scratch;
     TimeScope ts(c, 4, (double)Nx * Ny * batch * 32);
     hipError_t e = hh::launch_rhs2d(r, F, c->stream);
     if (e != hipSuccess) return fail(HH_ERR_DEVICE, "rhs2d launch failed: %s", hipGetErrorString(e));
@@ -840,23 +879,29 @@ extern "C" int hh_rhs3d(hh_plan *p, int order, double hx, double hy, double hz, 
     const int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz;
     const long E = (long)(Nx + 2) * (Ny + 2) * (Nz + 2);
     size_t wbytes = order == 6 ? (size_t)3 * E * batch * 16 : 0;
-    size_t tbytes = (size_t)(2 * Nz + 2) * 16;
+    const int nop = p->nop;
+    size_t tbytes = (size_t)(2 * Nz + 2) * nop * 16;
     if ((rc = ensure_scratch(c, wbytes + tbytes))) return rc;
-    // k^2 on the extended z grid and compact (k^2)_z (derivative_kz_4th_order)
-    std::vector<cd> tab(2 * (size_t)Nz + 2);
-    for (int l = 0; l < Nz + 2; l++) tab[l] = ld(k2e, l);
+    // tables [op]: k^2 on the extended z grid, then compact (k^2)_z
+    // (derivative_kz_4th_order, 4th_order_partials.c:296-340)
+    std::vector<cd> tab((2 * (size_t)Nz + 2) * nop);
     std::vector<cd> kz(Nz);
-    for (int l = 0; l < Nz; l++) kz[l] = 3.0 * (ld(k2e, l + 2) - ld(k2e, l)) / hz;
-    kz[0] -= ld(k2d, 0);
-    kz[Nz - 1] -= ld(k2d, 1);
-    compact_line(kz, 4.0);
-    for (int l = 0; l < Nz; l++) tab[Nz + 2 + l] = kz[l];
+    for (int o = 0; o < nop; o++) {
+        const double *ke = k2e + 2 * (size_t)o * (Nz + 2);
+        const double *kd = k2d ? k2d + 8 * (size_t)o : nullptr;
+        for (int l = 0; l < Nz + 2; l++) tab[(size_t)o * (Nz + 2) + l] = ld(ke, l);
+        for (int l = 0; l < Nz; l++) kz[l] = 3.0 * (ld(ke, l + 2) - ld(ke, l)) / hz;
+        kz[0] -= ld(kd, 0);
+        kz[Nz - 1] -= ld(kd, 1);
+        compact_line(kz, 4.0);
+        for (int l = 0; l < Nz; l++) tab[(size_t)nop * (Nz + 2) + (size_t)o * Nz + l] = kz[l];
+    }
     char *tb = (char *)c->scratch + wbytes;
     HIPCHK(hipMemcpyAsync(tb, tab.data(), tbytes, hipMemcpyHostToDevice, c->stream));
     hh::Rhs3D r;
     r.order = order; r.Nx = Nx; r.Ny = Ny; r.Nz = Nz; r.batch = batch; r.hx = hx; r.hy = hy; r.hz = hz;
     r.f_e = f_e; r.u_e = u_e; r.fxx_f = fxx_f; r.fyy_f = fyy_f; r.fzz_f = fzz_f;
-    r.k2e = tb; r.kz = tb + (size_t)(Nz + 2) * 16; r.coef = p->coef64; r.work = c->scratch;
+    r.k2e = tb; r.kz = tb + (size_t)(Nz + 2) * nop * 16; r.coef = p->coef64; r.nop = nop; r.work = c->scratch;
     TimeScope ts(c, 4, (double)Nx * Ny * Nz * batch * 32);
     hipError_t e = hh::launch_rhs3d(r, F, c->stream);
     if (e != hipSuccess) return fail(HH_ERR_DEVICE, "rhs3d launch failed: %s", hipGetErrorString(e));

@@ -40,6 +40,8 @@ struct TriHelm {
     long field;         // elements per right-hand side
     int batch;
     const void *coef;   // 2D: [q][6] (bm,am,b,a,bp,ap); 3D: [q][12] (a..dm)
+    int nop;            // operators: right-hand side b uses table b % nop
+    long cstride;       // complex entries between operator tables
     const double *lamx, *lamy;   // eigenvalues 2cos(k pi/(N+1))
     int W, P, mmax;     // block shape (modes per block, chunks per mode, rows per chunk)
 };
@@ -76,8 +78,9 @@ struct Rhs2D {
     const void *fyy_b;   // ring
     const void *k2c;     // Ny: k^2 at interior rows
     const void *k2y;     // Ny: compact (k^2)_y at interior rows
-    const void *coef;    // Ny x 6
-    void *work;          // 3 (Nx+2)(Ny+2)+Nx Ny per rhs
+    const void *coef;    // Ny x 6 per operator
+    int nop;             // right-hand side b uses operator b % nop
+    void *work;          // 2 (Nx+2)(Ny+2)+Nx Ny per rhs
 };
 hipError_t launch_rhs2d(const Rhs2D &r, void *F, hipStream_t st);
 
@@ -88,15 +91,17 @@ struct Rhs3D {
     const void *fxx_f, *fyy_f, *fzz_f;  // face arrays
     const void *k2e;              // Nz+2
     const void *kz;               // Nz: compact (k^2)_z
-    const void *coef;             // Nz x 12
+    const void *coef;             // Nz x 12 per operator
+    int nop;                      // right-hand side b uses operator b % nop
     void *work;                   // 3 extended fields per rhs
 };
 hipError_t launch_rhs3d(const Rhs3D &r, void *F, hipStream_t st);
 
 // res = A u - F (the reference residual.c), and the fused refinement pieces.
-hipError_t launch_residual2d(int Nx, int Ny, int batch, const void *coef, const void *u,
+// coef tables: right-hand side b uses coef + (b % nop) * (6 Ny | 12 Nz)
+hipError_t launch_residual2d(int Nx, int Ny, int batch, const void *coef, int nop, const void *u,
                              const void *F, void *res, int res_f32, hipStream_t st);
-hipError_t launch_residual3d(int Nx, int Ny, int Nz, int batch, const void *coef, const void *u,
+hipError_t launch_residual3d(int Nx, int Ny, int Nz, int batch, const void *coef, int nop, const void *u,
                              const void *F, void *res, int res_f32, hipStream_t st);
 // u += s*c (c in FP32 or FP64)
 hipError_t launch_axpy_update(long n, void *u, const void *c, int c_f32, double s, hipStream_t st);

@@ -49,6 +49,7 @@ struct Rhs2DArgs {
     int order, Nx, Ny;
     double hx, hy;
     const z2 *f_e, *u_e, *k2c, *k2y, *coef;
+    int nop;
     const z2 *work;
     long wstride;
 };
@@ -70,11 +71,12 @@ __global__ void k_rhs2d_combine(Rhs2DArgs a, long total, z2 *F) {
         z2 fxxxx = zs(za(zd(fxx[c0 - 1], zs(fxx[c0], 2.0)), fxx[c0 + 1]), 1.0 / hx2);
         z2 fxxyy = zs(za(zd(fxx[c0 - X], zs(fxx[c0], 2.0)), fxx[c0 + X]), 1.0 / hy2);
         z2 fyyyy = zs(za(zd(fyy[c0 - X], zs(fyy[c0], 2.0)), fyy[c0 + X]), 1.0 / hy2);
-        z2 t1 = zm(zd(zr(1.0), zs(a.k2c[j], hy2 / 20.0)), f[c0]);
+        const long oj = (long)(b % a.nop) * Ny + j;
+        z2 t1 = zm(zd(zr(1.0), zs(a.k2c[oj], hy2 / 20.0)), f[c0]);
         z2 t2 = zs(za(fxx[c0], fyy[c0]), hy2 / 12.0);
         z2 t3 = zs(za(fxxxx, fyyyy), hy4 / 360.0);
         z2 t4 = zs(fxxyy, hy4 / 90.0);
-        z2 t5 = zs(zm(a.k2y[j], fy[p]), hy4 / 60.0);
+        z2 t5 = zs(zm(a.k2y[oj], fy[p]), hy4 / 60.0);
         v = zs(za(za(za(za(t1, t2), t3), t4), t5), hy2);
     } else if (a.order == 4) {                                    // rhs.c:21-38
         z2 s4 = za(za(za(f[c0 - X], f[c0 + X]), f[c0 - 1]), f[c0 + 1]);
@@ -86,7 +88,7 @@ __global__ void k_rhs2d_combine(Rhs2DArgs a, long total, z2 *F) {
     // for every neighbour that lies on the boundary
     if (i == 0 || i == Nx - 1 || j == 0 || j == Ny - 1) {
         const z2 *u = a.u_e + b * XY;
-        const z2 *cq = a.coef + 6 * (long)j;  // bm, am, b, a, bp, ap
+        const z2 *cq = a.coef + 6 * ((long)(b % a.nop) * Ny + j);  // bm, am, b, a, bp, ap
         for (int dj = -1; dj <= 1; dj++)
             for (int di = -1; di <= 1; di++) {
                 int ii = i + di, jj = j + dj;
@@ -135,7 +137,7 @@ hipError_t launch_rhs2d(const Rhs2D &r, void *F, hipStream_t st) {
     Rhs2DArgs a;
     a.order = r.order; a.Nx = Nx; a.Ny = Ny; a.hx = r.hx; a.hy = r.hy;
     a.f_e = (const z2 *)r.f_e; a.u_e = (const z2 *)r.u_e; a.k2c = (const z2 *)r.k2c; a.k2y = (const z2 *)r.k2y;
-    a.coef = (const z2 *)r.coef; a.work = work; a.wstride = wstride;
+    a.coef = (const z2 *)r.coef; a.nop = r.nop > 0 ? r.nop : 1; a.work = work; a.wstride = wstride;
     long total = (long)Nx * Ny * r.batch;
     hipLaunchKernelGGL(k_rhs2d_combine, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, a, total, (z2 *)F);
     return hipGetLastError();
@@ -179,6 +181,7 @@ struct Rhs3DArgs {
     int order, Nx, Ny, Nz;
     double hx, hy, hz;
     const z2 *f_e, *u_e, *k2e, *kz, *coef;
+    int nop;
     const z2 *work;
     long wstride;
 };
@@ -203,11 +206,12 @@ __global__ void k_rhs3d_combine(Rhs3DArgs a, long total, z2 *F) {
         z2 fxxyy = zs(za(zd(fxx[c0 - X], zs(fxx[c0], 2.0)), fxx[c0 + X]), 1.0 / (a.hy * a.hy));
         z2 fxxzz = zs(za(zd(fxx[c0 - XY], zs(fxx[c0], 2.0)), fxx[c0 + XY]), 1.0 / h2);
         z2 fyyzz = zs(za(zd(fyy[c0 - XY], zs(fyy[c0], 2.0)), fyy[c0 + XY]), 1.0 / h2);
-        z2 first = zm(zd(zr(1.0), zs(a.k2e[l + 1], h2 / 20.0)), f[c0]);
+        const long op = b % a.nop;
+        z2 first = zm(zd(zr(1.0), zs(a.k2e[op * (Nz + 2) + l + 1], h2 / 20.0)), f[c0]);
         z2 second = zs(za(za(fxx[c0], fyy[c0]), fzz[c0]), h2 / 12.0);
         z2 third = zs(za(za(fxxxx, fyyyy), fzzzz), h2 * h2 / 360.0);
         z2 fourth = zs(za(za(fxxyy, fxxzz), fyyzz), h2 * h2 / 90.0);
-        z2 fifth = zs(zm(a.kz[l], zd(f[c0 + XY], f[c0 - XY])), h2 * hz / 120.0);
+        z2 fifth = zs(zm(a.kz[op * Nz + l], zd(f[c0 + XY], f[c0 - XY])), h2 * hz / 120.0);
         v = za(za(za(za(first, second), third), fourth), fifth);
     } else if (a.order == 4) {                                    // rhs.c:34-53
         z2 s6 = za(za(za(za(za(f[c0 - 1], f[c0 + 1]), f[c0 - X]), f[c0 + X]), f[c0 - XY]), f[c0 + XY]);
@@ -224,7 +228,7 @@ __global__ void k_rhs3d_combine(Rhs3DArgs a, long total, z2 *F) {
             if (a.order == 2) {
                 v = zd(v, u[g]);
             } else {
-                const z2 *cq = a.coef + 12 * (side == 0 ? 0 : (Nz - 1)) + (side == 0 ? 8 : 4);  // am.. or ap..
+                const z2 *cq = a.coef + 12 * ((long)(b % a.nop) * Nz + (side == 0 ? 0 : (Nz - 1))) + (side == 0 ? 8 : 4);
                 z2 corner = za(za(za(u[g - 1 - X], u[g + 1 - X]), u[g - 1 + X]), u[g + 1 + X]);
                 z2 xn = za(u[g - 1], u[g + 1]);
                 z2 yn = za(u[g - X], u[g + X]);
@@ -270,7 +274,7 @@ hipError_t launch_rhs3d(const Rhs3D &r, void *F, hipStream_t st) {
     Rhs3DArgs a;
     a.order = r.order; a.Nx = r.Nx; a.Ny = r.Ny; a.Nz = r.Nz; a.hx = r.hx; a.hy = r.hy; a.hz = r.hz;
     a.f_e = (const z2 *)r.f_e; a.u_e = (const z2 *)r.u_e; a.k2e = (const z2 *)r.k2e; a.kz = (const z2 *)r.kz;
-    a.coef = (const z2 *)r.coef; a.work = work; a.wstride = wstride;
+    a.coef = (const z2 *)r.coef; a.nop = r.nop > 0 ? r.nop : 1; a.work = work; a.wstride = wstride;
     long total = Nx * Ny * Nz * r.batch;
     hipLaunchKernelGGL(k_rhs3d_combine, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, a, total, (z2 *)F);
     return hipGetLastError();
@@ -278,14 +282,15 @@ hipError_t launch_rhs3d(const Rhs3D &r, void *F, hipStream_t st) {
 
 // ---------------------------------------------------------------- residual
 template <typename To>
-__global__ void k_residual2d(int Nx, int Ny, long total, const z2 *coef, const z2 *u, const z2 *F, C2<To> *res) {
+__global__ void k_residual2d(int Nx, int Ny, long total, const z2 *coef, int nop, const z2 *u, const z2 *F,
+                             C2<To> *res) {
     long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= total) return;
     const long Nxy = (long)Nx * Ny;
     const long b = id / Nxy, p = id % Nxy;
     const int i = (int)(p % Nx), j = (int)(p / Nx);
     const z2 *ub = u + b * Nxy;
-    const z2 *cq = coef + 6 * (long)j;  // bm, am, b, a, bp, ap
+    const z2 *cq = coef + 6 * ((long)(b % nop) * Ny + j);  // bm, am, b, a, bp, ap
     auto U = [&](int ii, int jj) -> z2 {
         return (ii >= 0 && ii < Nx && jj >= 0 && jj < Ny) ? ub[ii + (long)jj * Nx] : zr(0.0);
     };
@@ -296,7 +301,7 @@ __global__ void k_residual2d(int Nx, int Ny, long total, const z2 *coef, const z
 }
 
 template <typename To>
-__global__ void k_residual3d(int Nx, int Ny, int Nz, long total, const z2 *coef, const z2 *u, const z2 *F,
+__global__ void k_residual3d(int Nx, int Ny, int Nz, long total, const z2 *coef, int nop, const z2 *u, const z2 *F,
                              C2<To> *res) {
     long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= total) return;
@@ -312,7 +317,7 @@ __global__ void k_residual3d(int Nx, int Ny, int Nz, long total, const z2 *coef,
     // levels l-1 (am..dm), l (a..d), l+1 (ap..dp)
     const int off[3] = {8, 0, 4};
     for (int q = 0; q < 3; q++) {
-        const z2 *c = coef + 12 * (long)l + off[q];
+        const z2 *c = coef + 12 * ((long)(b % nop) * Nz + l) + off[q];
         int ll = l + q - 1;
         z2 corner = za(za(za(U(i + 1, j + 1, ll), U(i - 1, j + 1, ll)), U(i + 1, j - 1, ll)), U(i - 1, j - 1, ll));
         z2 xn = za(U(i + 1, j, ll), U(i - 1, j, ll));
@@ -323,28 +328,28 @@ __global__ void k_residual3d(int Nx, int Ny, int Nz, long total, const z2 *coef,
     res[id] = cconv<To, double>(zd(acc, F[id]));
 }
 
-hipError_t launch_residual2d(int Nx, int Ny, int batch, const void *coef, const void *u, const void *F, void *res,
-                             int res_f32, hipStream_t st) {
+hipError_t launch_residual2d(int Nx, int Ny, int batch, const void *coef, int nop, const void *u, const void *F,
+                             void *res, int res_f32, hipStream_t st) {
     long total = (long)Nx * Ny * batch;
     dim3 g((unsigned)((total + 255) / 256));
     if (res_f32)
-        hipLaunchKernelGGL(k_residual2d<float>, g, dim3(256), 0, st, Nx, Ny, total, (const z2 *)coef, (const z2 *)u,
-                           (const z2 *)F, (float2 *)res);
+        hipLaunchKernelGGL(k_residual2d<float>, g, dim3(256), 0, st, Nx, Ny, total, (const z2 *)coef, nop,
+                           (const z2 *)u, (const z2 *)F, (float2 *)res);
     else
-        hipLaunchKernelGGL(k_residual2d<double>, g, dim3(256), 0, st, Nx, Ny, total, (const z2 *)coef, (const z2 *)u,
-                           (const z2 *)F, (z2 *)res);
+        hipLaunchKernelGGL(k_residual2d<double>, g, dim3(256), 0, st, Nx, Ny, total, (const z2 *)coef, nop,
+                           (const z2 *)u, (const z2 *)F, (z2 *)res);
     return hipGetLastError();
 }
 
-hipError_t launch_residual3d(int Nx, int Ny, int Nz, int batch, const void *coef, const void *u, const void *F,
-                             void *res, int res_f32, hipStream_t st) {
+hipError_t launch_residual3d(int Nx, int Ny, int Nz, int batch, const void *coef, int nop, const void *u,
+                             const void *F, void *res, int res_f32, hipStream_t st) {
     long total = (long)Nx * Ny * Nz * batch;
     dim3 g((unsigned)((total + 255) / 256));
     if (res_f32)
-        hipLaunchKernelGGL(k_residual3d<float>, g, dim3(256), 0, st, Nx, Ny, Nz, total, (const z2 *)coef,
+        hipLaunchKernelGGL(k_residual3d<float>, g, dim3(256), 0, st, Nx, Ny, Nz, total, (const z2 *)coef, nop,
                            (const z2 *)u, (const z2 *)F, (float2 *)res);
     else
-        hipLaunchKernelGGL(k_residual3d<double>, g, dim3(256), 0, st, Nx, Ny, Nz, total, (const z2 *)coef,
+        hipLaunchKernelGGL(k_residual3d<double>, g, dim3(256), 0, st, Nx, Ny, Nz, total, (const z2 *)coef, nop,
                            (const z2 *)u, (const z2 *)F, (z2 *)res);
     return hipGetLastError();
 }

@@ -29,13 +29,17 @@ namespace hh {
 // (bm, am, b, a, bp, ap) with sub = bm lam + am etc.; 3D [q][12] =
 // (a, b, c, d, ap, bp, cp, dp, am, bm, cm, dm) with
 // dia = a lx ly + b lx + c ly + d.
+// With nop > 1 operators, right-hand side b uses the table at
+// coef + (b % nop) * cstride (one operator per problem of a batch).
 template <typename T> struct HelmBands {
-    int dim, n, Nx;
+    int dim, n, Nx, nop;
     const C2<T> *coef;
+    long cstride;
     const double *lamx, *lamy;
-    struct Mode { T l0, l1, l2; };
-    HH_DEV Mode mode(long m) const {
+    struct Mode { T l0, l1, l2; const C2<T> *cf; };
+    HH_DEV Mode mode(long m, int bt) const {
         Mode md;
+        md.cf = coef + (long)(bt % nop) * cstride;
         if (dim == 2) {
             md.l0 = (T)lamx[m]; md.l1 = 0; md.l2 = 0;
         } else {
@@ -47,13 +51,13 @@ template <typename T> struct HelmBands {
     }
     HH_DEV void band(const Mode &md, int q, C2<T> &sub, C2<T> &dia, C2<T> &sup) const {
         if (dim == 2) {
-            const C2<T> *c = coef + 6 * (long)q;
+            const C2<T> *c = md.cf + 6 * (long)q;
             T l = md.l0;
             sub = mk<T>(c[0].x * l + c[1].x, c[0].y * l + c[1].y);
             dia = mk<T>(c[2].x * l + c[3].x, c[2].y * l + c[3].y);
             sup = mk<T>(c[4].x * l + c[5].x, c[4].y * l + c[5].y);
         } else {
-            const C2<T> *c = coef + 12 * (long)q;
+            const C2<T> *c = md.cf + 12 * (long)q;
             T a = md.l0, b = md.l1, g = md.l2;
             dia = mk<T>(c[0].x * a + c[1].x * b + c[2].x * g + c[3].x, c[0].y * a + c[1].y * b + c[2].y * g + c[3].y);
             sup = mk<T>(c[4].x * a + c[5].x * b + c[6].x * g + c[7].x, c[4].y * a + c[5].y * b + c[6].y * g + c[7].y);
@@ -73,7 +77,7 @@ template <typename T, typename Ti, typename To> struct HelmProb {
     long field, S;
     struct Mode { typename HelmBands<T>::Mode bm; long m; };
     HH_DEV int n() const { return bands.n; }
-    HH_DEV Mode mode(long m) const { Mode md; md.bm = bands.mode(m); md.m = m; return md; }
+    HH_DEV Mode mode(long m, int bt) const { Mode md; md.bm = bands.mode(m, bt); md.m = m; return md; }
     HH_DEV void band(const Mode &md, int q, C2<T> &a, C2<T> &b, C2<T> &c) const { bands.band(md.bm, q, a, b, c); }
     HH_DEV C2<T> rhs(int b, const Mode &md, int q) const { return cconv<T, Ti>(in[b * field + md.m + (long)q * S]); }
     HH_DEV void store(int b, const Mode &md, int q, C2<T> x) const {
@@ -101,7 +105,7 @@ template <typename T> struct CompactProb {
     long oorig, oA, oB, oS;
     struct Mode { long in, bnd, out; };
     HH_DEV int n() const { return nn; }
-    HH_DEV Mode mode(long m) const {
+    HH_DEV Mode mode(long m, int) const {
         long a = m % mA, b = m / mA;
         Mode md;
         md.in = orig + a * sA + b * sB;
@@ -150,7 +154,7 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
     const int bt = blockIdx.y;
     const int s = (int)(((long)c * n) / P), e = (int)(((long)(c + 1) * n) / P) - 1;
     const int len = e - s + 1;
-    const auto md = prob.mode(valid ? m : 0);
+    const auto md = prob.mode(valid ? m : 0, bt);
 
     C2<T> ab[MMAX - 1], cb[MMAX - 1], db[MMAX - 1];
     C2<T> sub_s = mk<T>(0, 0), dia_s = mk<T>(1, 0), sup_s = mk<T>(0, 0), d_s = mk<T>(0, 0);
@@ -302,6 +306,8 @@ static hipError_t launch_tri_t(const TriHelm &t, const void *in, void *out, hipS
     pb.bands.dim = t.dim;
     pb.bands.Nx = t.Nx;
     pb.bands.coef = (const C2<T> *)t.coef;
+    pb.bands.nop = t.nop > 0 ? t.nop : 1;
+    pb.bands.cstride = t.cstride;
     pb.bands.lamx = t.lamx;
     pb.bands.lamy = t.lamy;
     pb.in = (const C2<Ti> *)in;

@@ -174,6 +174,19 @@ def coefficients3d(order, Nx, Ny, Nz, hx, hy, hz, k2e, k2d=None):
 
 
 # ------------------------------------------------------------------ plans
+def _operators(coef, nvec, n):
+    """Normalise one operator (nvec vectors of length n) or a list of them."""
+    a = coef
+    if isinstance(a, np.ndarray) and a.ndim == 3:
+        return [[_c128(v, n) for v in op] for op in a]
+    if isinstance(a, np.ndarray) and a.ndim == 2:
+        return [[_c128(v, n) for v in a]]
+    a = list(a)
+    if len(a) == nvec and np.ndim(a[0]) == 1:
+        return [[_c128(v, n) for v in a]]
+    return [[_c128(v, n) for v in (op.reshape(nvec, n) if np.ndim(op) == 1 else op)] for op in a]
+
+
 class Plan:
     dim = 0
 
@@ -240,15 +253,22 @@ class Plan:
 
 
 class Plan2D(Plan):
-    """2D operator: coefficient vectors [bp, ap, b, a, bm, am] of length Ny."""
+    """2D operator(s): coefficient vectors [bp, ap, b, a, bm, am] of length Ny.
+
+    `coef` is one operator (six vectors) or a list of operators; with nop
+    operators, right-hand side b of a solve uses operator b % nop."""
     dim = 2
 
     def __init__(self, ctx, Nx, Ny, coef, precision=PREC_F64):
         super().__init__(ctx)
         self.Nx, self.Ny = Nx, Ny
-        self.coef = [_c128(c, Ny) for c in coef]
+        ops = _operators(coef, 6, Ny)
+        self.nop = len(ops)
+        self.coef = ops[0]
+        self.ops = ops
+        packed = np.ascontiguousarray(np.stack([np.stack(o) for o in ops]))
         h = ctypes.c_void_p()
-        _check(lib().hh_plan2d_create(ctx.h, Nx, Ny, *[_hptr(c) for c in self.coef], precision, ctypes.byref(h)))
+        _check(lib().hh_plan2d_create_multi(ctx.h, Nx, Ny, self.nop, _hptr(packed), precision, ctypes.byref(h)))
         self.h = h
         self.precision = precision
 
@@ -275,12 +295,16 @@ class Plan3D(Plan):
     def __init__(self, ctx, Nx, Ny, Nz, coef, precision=PREC_F64):
         super().__init__(ctx)
         self.Nx, self.Ny, self.Nz = Nx, Ny, Nz
-        if len(coef) == 12 * Nz and np.ndim(coef) == 1:
+        if np.ndim(coef) == 1 and len(coef) == 12 * Nz:
             coef = np.asarray(coef).reshape(12, Nz)
-        self.coef = [_c128(c, Nz) for c in coef]
-        arr = (ctypes.c_void_p * 12)(*[c.ctypes.data for c in self.coef])
+        ops = _operators(coef, 12, Nz)
+        self.nop = len(ops)
+        self.coef = ops[0]
+        self.ops = ops
+        packed = np.ascontiguousarray(np.stack([np.stack(o) for o in ops]))
         h = ctypes.c_void_p()
-        _check(lib().hh_plan3d_create(ctx.h, Nx, Ny, Nz, arr, precision, ctypes.byref(h)))
+        _check(lib().hh_plan3d_create_multi(ctx.h, Nx, Ny, Nz, self.nop, _hptr(packed), precision,
+                                            ctypes.byref(h)))
         self.h = h
         self.precision = precision
 

@@ -63,6 +63,18 @@ void forwardDST(System sys, DSTN dst, double _Complex *rhs, double _Complex *rha
 void reverseDST(System sys, DSTN dst, double _Complex *xhat, double _Complex *sol, fftw_plan plan, double *in,
                 fftw_complex *out, fftw_plan plan2, double *in2, fftw_complex *out2);
 
+/* 2D_Program/headers/prototypes.h:23, main_functions/rhs.c:100-150 with the
+ * partial derivatives: fills sys.k, sys.sol_analytic and the compact
+ * right-hand side sys.rhs (Dirichlet closure included) for sys.order.
+ * Samples the program's own user-input functions k, k2_y, f, f_y, f_xx,
+ * f_yy and analytic_solution (2D_Program/user_input.c) on the host and
+ * assembles on the GPU.  Needs the stencil vectors from coefficients(). */
+void rhs(System sys);
+
+/* 2D_Program/main_functions/residual.c:3-34: sys.res = A sys.sol - sys.rhs
+ * (9-point operator, zero ghosts), sys.error = sys.sol - sys.sol_analytic. */
+void residual(System sys);
+
 /* Status of the last call on this thread (HH_OK = 0); no reference
  * counterpart, the reference's functions return void. */
 int hh_ref2d_last_status(void);

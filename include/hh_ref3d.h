@@ -51,6 +51,18 @@ void solver(System sys);
 void DST(int Nx, int Ny, double *b_2D, double *bhat, fftw_plan p1, double *in1, fftw_complex *out1, fftw_plan p2,
          double *in2, fftw_complex *out2);
 
+/* 3D_Program/main_functions/rhs.c:198-294 with the partial derivatives:
+ * the reference problem u = sin(gamma x) sin(beta y) exp(-k(z)/C),
+ * k(z) = A - B sin(Cz) (parameters sys.A, B, C, gamma, beta and the
+ * background sys.k_bg_ext, sys.k2_bg_ext), sampled on the host as the
+ * reference's user-input blocks do, assembled on the GPU; fills
+ * sys.sol_analytic and sys.rhs.  Needs the stencil vectors. */
+void rhs(System sys);
+
+/* 3D_Program/main_functions/residual.c:42-96: sys.res = A sys.sol - sys.rhs
+ * (27-point operator), sys.error = sys.sol - sys.sol_analytic. */
+void residual(System sys);
+
 /* Status of the last call on this thread (0 = ok); no reference counterpart. */
 int hh_ref3d_last_status(void);
 

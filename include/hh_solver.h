@@ -43,7 +43,8 @@ const char *hh_status_string(int status);
 /* ---- context: device, stream, twiddle/DST tables, scratch ---- */
 int hh_context_create(int device, hh_context **ctx);
 int hh_context_destroy(hh_context *ctx);
-/* run on a caller stream (hipStream_t); NULL restores the context's own */
+/* run on a caller stream (hipStream_t); NULL selects the HIP default (null)
+ * stream.  A new context runs on a private non-blocking stream. */
 int hh_context_set_stream(hh_context *ctx, void *stream);
 void *hh_context_get_stream(hh_context *ctx);
 int hh_context_synchronize(hh_context *ctx);
@@ -86,6 +87,16 @@ int hh_plan2d_create(hh_context *ctx, int Nx, int Ny, const double *bp, const do
                      int precision, hh_plan **plan);
 int hh_plan3d_create(hh_context *ctx, int Nx, int Ny, int Nz, const double *const coef[12],
                      int precision, hh_plan **plan);
+/* One operator per problem of a batch (no reference counterpart: the
+ * reference solves one problem per call).  coef holds nop operators back to
+ * back, each as the six (2D: bp, ap, b, a, bm, am, length Ny) or twelve (3D:
+ * a..dm, length Nz) complex vectors; right-hand side b of a solve uses
+ * operator b % nop.  All operators share the grid. */
+int hh_plan2d_create_multi(hh_context *ctx, int Nx, int Ny, int nop, const double *coef, int precision,
+                           hh_plan **plan);
+int hh_plan3d_create_multi(hh_context *ctx, int Nx, int Ny, int Nz, int nop, const double *coef, int precision,
+                           hh_plan **plan);
+int hh_plan_num_operators(const hh_plan *plan);
 int hh_plan_destroy(hh_plan *plan);
 hh_context *hh_plan_context(const hh_plan *plan);
 /* grid sizes of a plan: dims[0..2] = Nx, Ny, Nz (Nz = 1 in 2D); returns 2 or 3 */
@@ -129,7 +140,8 @@ int hh_dst2d_layers(hh_context *ctx, int Nx, int Ny, long nlayers, const void *d
  * f_y at (x_i, y0) then (x_i, y1) (2*Nx); fxx_b, fyy_b = f_xx, f_yy on the
  * ghost ring: bottom row, top row, left column, right column
  * (2(Nx+2)+2(Ny+2)).  Host, complex: k2e = k^2 at y0 + j*hy (Ny+2), k2yb =
- * (k^2)_y at y0, y1 (2).  Orders 2 and 4 ignore fy_b, fxx_b, fyy_b, k2yb. */
+ * (k^2)_y at y0, y1 (2), each repeated per operator of a multi-operator
+ * plan.  Orders 2 and 4 ignore fy_b, fxx_b, fyy_b, k2yb. */
 int hh_rhs2d(hh_plan *plan, int order, double hx, double hy, const void *d_f_e, const void *d_u_e,
              const void *d_fy_b, const void *d_fxx_b, const void *d_fyy_b, const double *k2e,
              const double *k2yb, void *d_F, int batch);
