@@ -1,0 +1,137 @@
+"""Autotuner of the kernel launch shapes.
+
+The tridiagonal kernel's block shape (W neighbouring modes per block, rows
+per thread, hence chunks per mode and the depth of the reduced system that
+parallel cyclic reduction solves) is chosen by timing the candidates on the
+device with the library's own kernel timers (HIP events on the launch
+stream).  Results are kept in a JSON table keyed by
+(dimension, system length, modes, batch bucket, precision) so that a shape
+does not depend on what else a batch contains; plans created through
+`tuned_plan2d` / `tuned_plan3d` look the table up.
+
+    python -m hackathon_amd.autotune --dim 2 --N 1023 --batch 64
+"""
+import argparse
+import ctypes
+import json
+import os
+
+import numpy as np
+
+from . import helmholtz as H
+
+TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+CANDIDATES = [(w, m) for m in (4, 8, 16) for w in (1, 2, 4, 8, 16, 32)]
+
+
+def _bucket(batch):
+    b = 1
+    while b < batch:
+        b *= 2
+    return b
+
+
+def key(dim, n, nmodes, batch, precision):
+    return f"{dim}:{n}:{nmodes}:{_bucket(batch)}:{precision}"
+
+
+def load_table(path=TABLE):
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return {}
+
+
+def save_table(table, path=TABLE):
+    with open(path, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+
+
+def time_tri(plan, rhs, out, reps=3):
+    """Average tridiagonal-kernel milliseconds of one solve of rhs."""
+    lib = H.lib()
+    ctx = plan.ctx
+    plan.solve(rhs, out=out)  # warm-up (twiddles, scratch, code objects)
+    lib.hh_context_reset_timing(ctx.h)
+    lib.hh_context_enable_timing(ctx.h, 1)
+    for _ in range(reps):
+        plan.solve(rhs, out=out)
+    lib.hh_context_enable_timing(ctx.h, 0)
+    ms = (ctypes.c_double * 5)()
+    nl = (ctypes.c_long * 5)()
+    lib.hh_context_read_timing(ctx.h, ms, nl, None)
+    return ms[2] / max(nl[2], 1), [ms[k] / max(nl[k], 1) for k in range(5)]
+
+
+def tune(plan, rhs, out, candidates=CANDIDATES, reps=3, verbose=False):
+    """Try every shape that fits; leave the fastest set on the plan."""
+    lib = H.lib()
+    results = []
+    for w, m in candidates:
+        if lib.hh_plan_set_tri_shape(plan.h, w, m) != 0:
+            continue
+        t, _ = time_tri(plan, rhs, out, reps)
+        results.append((t, w, m))
+        if verbose:
+            print(f"  W={w:3d} rows={m:3d}: {1e3 * t:9.1f} us", flush=True)
+    results.sort()
+    best = results[0]
+    lib.hh_plan_set_tri_shape(plan.h, best[1], best[2])
+    return {"W": best[1], "rows": best[2], "us": 1e3 * best[0],
+            "all": [{"W": w, "rows": m, "us": 1e3 * t} for t, w, m in results]}
+
+
+def apply_tuned(plan, batch, table=None):
+    """Set the tabulated shape on a plan, if the table has one."""
+    table = load_table() if table is None else table
+    n = plan.Ny if plan.dim == 2 else plan.Nz
+    nmodes = plan.Nx if plan.dim == 2 else plan.Nx * plan.Ny
+    e = table.get(key(plan.dim, n, nmodes, batch, plan.precision))
+    if e:
+        H._check(H.lib().hh_plan_set_tri_shape(plan.h, e["W"], e["rows"]))
+    return e
+
+
+def main():
+    import torch
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--dim", type=int, default=2)
+    ap.add_argument("--N", type=int, default=1023)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--precision", type=int, default=H.PREC_F64)
+    ap.add_argument("--k", type=float, default=50.0)
+    ap.add_argument("--save", action="store_true")
+    args = ap.parse_args()
+    ctx = H.Context(0)
+    N = args.N
+    h = 1.0 / (N + 1)
+    if args.dim == 2:
+        coef = H.coefficients2d(6, N, N, h, h, np.full(N + 2, args.k ** 2, np.complex128))
+        plan = H.Plan2D(ctx, N, N, coef, precision=args.precision)
+        shape = (args.batch, N, N)
+    else:
+        coef = H.coefficients3d(6, N, N, N, h, h, h, np.full(N + 2, args.k ** 2, np.complex128))
+        plan = H.Plan3D(ctx, N, N, N, coef, precision=args.precision)
+        shape = (args.batch, N, N, N)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    rhs = torch.randn(shape, dtype=torch.complex128, device="cuda", generator=g)
+    out = torch.empty_like(rhs)
+    print(f"tridiagonal shapes, dim={args.dim} N={N} batch={args.batch} precision={args.precision}", flush=True)
+    res = tune(plan, rhs, out, verbose=True)
+    print(json.dumps({k: v for k, v in res.items() if k != "all"}), flush=True)
+    for algo in (0, 1):
+        ctx.set_dst_algorithm(algo)
+        _, per = time_tri(plan, rhs, out)
+        print(f"DST algorithm {algo}: {1e3 * per[0]:9.1f} us per launch", flush=True)
+    ctx.set_dst_algorithm(0)
+    if args.save:
+        table = load_table()
+        n = N
+        nmodes = N if args.dim == 2 else N * N
+        table[key(args.dim, n, nmodes, args.batch, args.precision)] = {"W": res["W"], "rows": res["rows"],
+                                                                       "us": round(res["us"], 1)}
+        save_table(table)
+
+
+if __name__ == "__main__":
+    main()

@@ -48,6 +48,8 @@ struct hh_context {
     hipStream_t own = nullptr, stream = nullptr;
     std::map<int, void *> tw64, tw32;   // twiddles by FFT length
     std::map<int, void *> S64, S32;     // dense DST matrices by N
+    std::map<int, void *> th64, th32;   // half-length transform tables by N+1
+    int dst_algo = 0;                   // 0: (N+1)-point transform, 1: odd extension
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     double *d_norm = nullptr;
@@ -97,6 +99,7 @@ struct hh_plan {
     std::vector<cd> hcoef;                       // host copy of the band table
     double tol = 1e-12;
     int max_iter = 6;
+    int tri_W = 0, tri_mm = 0;                   // tuned tridiagonal block shape (0 = heuristic)
     int last_sweeps = 0;
     double last_rel = 0;
 };
@@ -197,16 +200,48 @@ static int get_dst_matrix(hh_context *c, int N, int f32, void **out) {
     return HH_OK;
 }
 
+// Table of the half-length transform: M = N+1 twiddles e^{-2 pi i q/M}, then
+// the M reals sin(pi j/M) (packed into (M+1)/2 complex slots).
+static int get_half_table(hh_context *c, int N, int f32, void **out) {
+    const int M = N + 1;
+    auto &m = f32 ? c->th32 : c->th64;
+    auto it = m.find(M);
+    if (it != m.end()) { *out = it->second; return HH_OK; }
+    std::vector<cd> tw;
+    twiddles(M, tw);
+    const int ns = (M + 1) / 2;
+    std::vector<double> h(2 * ((size_t)M + ns), 0.0);
+    for (int q = 0; q < M; q++) { h[2 * q] = tw[q].real(); h[2 * q + 1] = tw[q].imag(); }
+    for (int j = 0; j < M; j++)
+        h[2 * (size_t)M + j] = (double)sinl(3.14159265358979323846264338327950288L * (long double)j / (long double)M);
+    size_t n = h.size();
+    void *d = nullptr;
+    int rc = dev_alloc(c, &d, n * (f32 ? 4 : 8));
+    if (rc) return rc;
+    if (f32) {
+        std::vector<float> hf(h.begin(), h.end());
+        HIPCHK(hipMemcpy(d, hf.data(), n * 4, hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(hipMemcpy(d, h.data(), n * 8, hipMemcpyHostToDevice));
+    }
+    m[M] = d;
+    *out = d;
+    return HH_OK;
+}
+
 // One DST pass over lines; in-place when the FFT path applies.
 static int dst_pass(hh_context *c, const hh::Lines &ln, int ti, int tc, int to, const void *in, void *out) {
     void *tw = nullptr, *S = nullptr;
     int rc;
-    if (hh::fft_supported(ln.N)) rc = get_twiddles(c, ln.N, tc, &tw);
+    const int algo = c->dst_algo;
+    if (hh::fft_supported(ln.N)) rc = algo == 0 ? get_half_table(c, ln.N, tc, &tw) : get_twiddles(c, ln.N, tc, &tw);
     else rc = get_dst_matrix(c, ln.N, tc, &S);
     if (rc) return rc;
     const double elems = (double)ln.N * ln.ncols * ln.nplanes;
     TimeScope ts(c, tw ? 0 : 1, elems * ((ti ? 8 : 16) + (to ? 8 : 16)));
-    hipError_t e = hh::launch_dst(ln, hh::DstTypes{ti, tc, to}, in, out, tw, S, c->stream);
+    hh::DstTypes ty{ti, tc, to};
+    ty.algo = algo;
+    hipError_t e = hh::launch_dst(ln, ty, in, out, tw, S, c->stream);
     if (e != hipSuccess) return fail(HH_ERR_DEVICE, "DST launch (N=%d) failed: %s", ln.N, hipGetErrorString(e));
     return HH_OK;
 }
@@ -272,6 +307,8 @@ extern "C" int hh_context_destroy(hh_context *c) {
     for (auto &kv : c->tw32) (void)hipFree(kv.second);
     for (auto &kv : c->S64) (void)hipFree(kv.second);
     for (auto &kv : c->S32) (void)hipFree(kv.second);
+    for (auto &kv : c->th64) (void)hipFree(kv.second);
+    for (auto &kv : c->th32) (void)hipFree(kv.second);
     if (c->scratch) (void)hipFree(c->scratch);
     for (auto &r : c->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto e : c->pool) (void)hipEventDestroy(e);
@@ -318,6 +355,14 @@ static int collect_timing(hh_context *c) {
         c->pool.push_back(r.b);
     }
     c->recs.clear();
+    return HH_OK;
+}
+
+extern "C" int hh_context_set_dst_algorithm(hh_context *c, int algo) {
+    if (!c) return fail(HH_ERR_ARG, "null context");
+    if (algo != 0 && algo != 1) return fail(HH_ERR_ARG, "DST algorithm must be 0 or 1 (got %d)", algo);
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->dst_algo = algo;
     return HH_OK;
 }
 
@@ -613,6 +658,39 @@ extern "C" int hh_plan_set_refine(hh_plan *p, double tol, int max_iter) {
     return HH_OK;
 }
 
+extern "C" int hh_plan_set_tri_shape(hh_plan *p, int W, int rows_per_thread) {
+    if (!p) return fail(HH_ERR_ARG, "null plan");
+    if (W == 0 && rows_per_thread == 0) { p->tri_W = p->tri_mm = 0; return HH_OK; }
+    int n = p->dim == 2 ? p->Ny : p->Nz;
+    if (W < 1 || W > 64 || (rows_per_thread != 4 && rows_per_thread != 8 && rows_per_thread != 16))
+        return fail(HH_ERR_ARG, "tridiagonal shape W=%d rows=%d not available", W, rows_per_thread);
+    int P = (n + rows_per_thread - 1) / rows_per_thread;
+    if (P > n / 2) P = n / 2;
+    if (P < 1 || W * P > 1024 || (n + P - 1) / P > rows_per_thread)
+        return fail(HH_ERR_UNSUPPORTED, "tridiagonal shape W=%d rows=%d does not fit n=%d", W, rows_per_thread, n);
+    p->tri_W = W;
+    p->tri_mm = rows_per_thread;
+    return HH_OK;
+}
+
+extern "C" int hh_plan_get_tri_shape(const hh_plan *p, int *W, int *P, int *rows_per_thread) {
+    if (!p) return fail(HH_ERR_ARG, "null plan");
+    hh::TriHelm t;
+    int n = p->dim == 2 ? p->Ny : p->Nz;
+    long nmodes = p->dim == 2 ? p->Nx : (long)p->Nx * p->Ny;
+    hh::tri_shape(n, nmodes, p->prec == HH_PREC_F64 ? hh::F64 : hh::F64, &t.W, &t.P, &t.mmax);
+    if (p->tri_W > 0) {
+        t.W = p->tri_W;
+        t.mmax = p->tri_mm;
+        t.P = (n + t.mmax - 1) / t.mmax;
+        if (t.P > n / 2) t.P = n / 2;
+    }
+    if (W) *W = t.W;
+    if (P) *P = t.P;
+    if (rows_per_thread) *rows_per_thread = t.mmax;
+    return HH_OK;
+}
+
 extern "C" int hh_plan_last_refine(const hh_plan *p, int *sweeps, double *rel) {
     if (!p) return fail(HH_ERR_ARG, "null plan");
     if (sweeps) *sweeps = p->last_sweeps;
@@ -642,6 +720,12 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
     hh::tri_shape(n, nmodes, tri_comp == 0 && tc == 0 ? hh::F64 : hh::F32, &t.W, &t.P, &t.mmax);
     if (tri_comp == 0 && tc == 1) {   // FP64 elimination on FP32 storage: FP64 register budget
         hh::tri_shape(n, nmodes, hh::F64, &t.W, &t.P, &t.mmax);
+    }
+    if (p->tri_W > 0 && p->tri_mm > 0) {   // shape chosen by the autotuner
+        t.W = p->tri_W;
+        t.mmax = p->tri_mm;
+        t.P = (n + t.mmax - 1) / t.mmax;
+        if (t.P > n / 2) t.P = n / 2;
     }
     auto tri = [&](void *buf) -> int {
         TimeScope ts(c, 2, (double)t.field * batch * (tc ? 16 : 32));

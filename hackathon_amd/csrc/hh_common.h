@@ -33,10 +33,27 @@ template <typename T> HH_DEV C2<T> cfms(C2<T> a, C2<T> b, C2<T> c) {
 template <typename T> HH_DEV C2<T> cmul_mi(C2<T> a) { return mk<T>(a.y, -a.x); }
 // multiply by +i
 template <typename T> HH_DEV C2<T> cmul_i(C2<T> a) { return mk<T>(-a.y, a.x); }
+// Reciprocal of a positive real: hardware estimate (v_rcp_f64 / v_rcp_f32)
+// refined by Newton steps in residual form, e = 1 - n r, r += r e (fused).
+// Each step squares the relative error; two steps take an estimate good to
+// 2^-14 or better to the last bit of FP64, one step suffices for FP32.  The
+// quotient is not the correctly rounded one but stays within an ulp of it.
+HH_DEV double rcp_nr(double n) {
+    double r = __builtin_amdgcn_rcp(n);
+    double e = __builtin_fma(-n, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-n, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+HH_DEV float rcp_nr(float n) {
+    float r = __builtin_amdgcn_rcpf(n);
+    float e = __builtin_fmaf(-n, r, 1.0f);
+    return __builtin_fmaf(r, e, r);
+}
 // 1/z = conj(z)/|z|^2
 template <typename T> HH_DEV C2<T> crcp(C2<T> z) {
     T n = z.x * z.x + z.y * z.y;
-    T r = (T)1 / n;
+    T r = rcp_nr(n);
     return mk<T>(z.x * r, -z.y * r);
 }
 template <typename T> HH_DEV T cabs2(C2<T> z) { return z.x * z.x + z.y * z.y; }

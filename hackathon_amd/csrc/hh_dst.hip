@@ -106,6 +106,171 @@ __global__ __launch_bounds__(dst_cfg<L>::THREADS) void k_dst_fft_cols(
     }
 }
 
+// ------------------------------------------------------------------------
+// Half-length DST-I.  With M = N+1 and x_0 = x_M = 0, the row
+//   w_j = sin(pi j/M)(x_j + x_{M-j}) + (x_j - x_{M-j})/2,  j = 1..M-1, w_0 = 0
+// has the M-point transform F_k = sum_j w_j e^{-2 pi i jk/M} with
+//   S_{2k}   = (i/2)(F_k - F_{M-k})
+//   S_{2k+1} = S_{2k-1} + (F_k + F_{M-k})/2,   S_{-1} = -S_1
+// where S_m = sum_j x_j sin(pi jm/M).  (The symmetric part of w carries
+// cos-weighted differences of odd outputs, the antisymmetric part the even
+// outputs; real and imaginary parts of x ride through the same complex
+// transform because every weight is real.)  This halves the FFT length and
+// the LDS of the odd-extension transform; the odd outputs need a prefix sum
+// over M/2 terms, done with wave shuffles inside a line and through LDS
+// across the waves of a line.
+template <int M> struct half_cfg {
+    static constexpr int P = fft_shape<M>::P;
+    static constexpr int LPB = P >= 256 ? 1 : 256 / P;
+    static constexpr int THREADS = P * LPB;
+    static constexpr int LP = fft_shape<M>::LP;
+    static constexpr int K = (M / 2) / P;   // pairs (F_k, F_{M-k}) per thread, M >= 2P
+};
+
+// Post-processing of one line whose transform F sits in (sre, sim); writes the
+// scaled outputs X_m = sqrt(2/M) S_m (m = 1..M-1) back to positions m of the
+// same buffer.  tot: LDS scratch of one complex per wave of the line.
+template <typename T, int M>
+HH_DEV void half_post(T *sre, T *sim, T *tot_re, T *tot_im, int t, T scale) {
+    constexpr int P = half_cfg<M>::P, K = half_cfg<M>::K;
+    C2<T> se[K], so[K];
+    C2<T> run = mk<T>(0, 0);
+#pragma unroll
+    for (int u = 0; u < K; u++) {
+        const int k = t * K + u;
+        const int km = (M - k) & (M - 1);
+        const C2<T> fk = mk<T>(sre[lds_pad(k)], sim[lds_pad(k)]);
+        const C2<T> fm = mk<T>(sre[lds_pad(km)], sim[lds_pad(km)]);
+        const C2<T> dif = csub<T>(fk, fm);
+        se[u] = mk<T>(-(T)0.5 * dif.y, (T)0.5 * dif.x);            // (i/2)(F_k - F_{M-k})
+        run = cadd<T>(run, cscale<T>(cadd<T>(fk, fm), (T)0.5));   // running sum of D_k
+        so[u] = run;                                              // inclusive prefix within the thread
+    }
+    // exclusive scan of the thread totals across the line
+    C2<T> excl = mk<T>(0, 0);
+    {
+        constexpr int SEG = P < 64 ? P : 64;
+        const int lane = t % SEG;
+        C2<T> inc = run;
+#pragma unroll
+        for (int d = 1; d < SEG; d <<= 1) {
+            T ox = __shfl_up(inc.x, d, SEG), oy = __shfl_up(inc.y, d, SEG);
+            if (lane >= d) { inc.x += ox; inc.y += oy; }
+        }
+        excl = csub<T>(inc, run);
+        if constexpr (P > 64) {
+            const int wv = t / 64;
+            if (lane == 63) { tot_re[wv] = inc.x; tot_im[wv] = inc.y; }
+            __syncthreads();
+            for (int q = 0; q < wv; q++) { excl.x += tot_re[q]; excl.y += tot_im[q]; }
+        }
+    }
+    // D_0 / 2 = S_1: the first thread's first D is F_0 (F_M = F_0)
+    const C2<T> f0 = mk<T>(sre[lds_pad(0)], sim[lds_pad(0)]);
+    const C2<T> s1 = cscale<T>(f0, (T)0.5);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < K; u++) {
+        const int k = t * K + u;
+        // S_{2k+1} = S_{-1} + sum_{k' <= k} D_k' = -S_1 + prefix
+        const C2<T> sodd = csub<T>(cadd<T>(so[u], excl), s1);
+        if (k > 0) {
+            sre[lds_pad(2 * k)] = se[u].x * scale;
+            sim[lds_pad(2 * k)] = se[u].y * scale;
+        }
+        sre[lds_pad(2 * k + 1)] = sodd.x * scale;
+        sim[lds_pad(2 * k + 1)] = sodd.y * scale;
+    }
+    __syncthreads();
+}
+
+// rows (contiguous lines); aux = [twiddles of length M | sin(pi j/M), j < M]
+template <typename Ti, typename T, typename To, int M>
+__global__ __launch_bounds__(half_cfg<M>::THREADS) void k_dst_half_rows(const C2<Ti> *in, C2<To> *out, long nrows,
+                                                                       T scale, const C2<T> *__restrict__ aux) {
+    constexpr int P = half_cfg<M>::P, LPB = half_cfg<M>::LPB, LP = half_cfg<M>::LP, N = M - 1, NT = half_cfg<M>::THREADS;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *smem = reinterpret_cast<T *>(smem_raw);
+    T *tot = smem + 2 * LPB * LP;   // per line: 2 * (P/64 + 1) reals
+    const int li = threadIdx.x / P, t = threadIdx.x % P;
+    T *sre = smem + (2 * li) * LP, *sim = smem + (2 * li + 1) * LP;
+    const long row0 = (long)blockIdx.x * LPB;
+    const long row = row0 + li;
+    const bool valid = row < nrows;
+    const C2<Ti> *x = in + (valid ? row : 0) * N;
+    const T *sn = reinterpret_cast<const T *>(aux + M);
+    auto get = [&](int n) -> C2<T> {
+        if (!valid || n == 0) return mk<T>(0, 0);
+        const C2<T> a = cconv<T, Ti>(x[n - 1]), b = cconv<T, Ti>(x[M - n - 1]);
+        const T s = sn[n];
+        return mk<T>(s * (a.x + b.x) + (T)0.5 * (a.x - b.x), s * (a.y + b.y) + (T)0.5 * (a.y - b.y));
+    };
+    auto put = [&](int k, C2<T> z) {
+        sre[lds_pad(k)] = z.x;
+        sim[lds_pad(k)] = z.y;
+    };
+    fft_line<T, M, true>(sre, sim, aux, t, get, put);
+    __syncthreads();
+    constexpr int NW = P > 64 ? P / 64 : 1;
+    half_post<T, M>(sre, sim, tot + li * 2 * NW, tot + li * 2 * NW + NW, t, scale);
+    // coalesced store of the block's rows (contiguous in memory)
+    const long nr = nrows - row0 < LPB ? nrows - row0 : LPB;
+    C2<To> *y = out + row0 * N;
+    for (int e = threadIdx.x; e < nr * N; e += NT) {
+        const int l = e / N, m = e % N + 1;
+        const T *lre = smem + (2 * l) * LP, *lim = smem + (2 * l + 1) * LP;
+        y[e] = cconv<To, T>(mk<T>(lre[lds_pad(m)], lim[lds_pad(m)]));
+    }
+}
+
+template <typename Ti, typename T, typename To, int M>
+__global__ __launch_bounds__(half_cfg<M>::THREADS) void k_dst_half_cols(const C2<Ti> *in, C2<To> *out, long ncols,
+                                                                       long ntiles, long plane_stride, long elem_stride,
+                                                                       T scale, const C2<T> *__restrict__ aux) {
+    constexpr int P = half_cfg<M>::P, LPB = half_cfg<M>::LPB, LP = half_cfg<M>::LP, N = M - 1, NT = half_cfg<M>::THREADS;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *smem = reinterpret_cast<T *>(smem_raw);
+    T *tot = smem + 2 * LPB * LP;
+    const long tile = blockIdx.x % ntiles, plane = blockIdx.x / ntiles;
+    const long c0 = tile * LPB;
+    const long base = plane * plane_stride + c0;
+    for (int e = threadIdx.x; e < N * LPB; e += NT) {
+        int c = e % LPB, q = e / LPB;
+        T *lre = smem + (2 * c) * LP, *lim = smem + (2 * c + 1) * LP;
+        C2<T> v = mk<T>(0, 0);
+        if (c0 + c < ncols) v = cconv<T, Ti>(in[base + c + q * elem_stride]);
+        lre[lds_pad(q + 1)] = v.x;
+        lim[lds_pad(q + 1)] = v.y;
+    }
+    __syncthreads();
+    const int li = threadIdx.x / P, t = threadIdx.x % P;
+    T *sre = smem + (2 * li) * LP, *sim = smem + (2 * li + 1) * LP;
+    const T *sn = reinterpret_cast<const T *>(aux + M);
+    auto get = [&](int n) -> C2<T> {
+        if (n == 0) return mk<T>(0, 0);
+        const int nb = M - n;
+        const T ar = sre[lds_pad(n)], ai = sim[lds_pad(n)];
+        const T br = nb < M ? sre[lds_pad(nb)] : (T)0, bi = nb < M ? sim[lds_pad(nb)] : (T)0;
+        const T s = sn[n];
+        return mk<T>(s * (ar + br) + (T)0.5 * (ar - br), s * (ai + bi) + (T)0.5 * (ai - bi));
+    };
+    auto put = [&](int k, C2<T> z) {
+        sre[lds_pad(k)] = z.x;
+        sim[lds_pad(k)] = z.y;
+    };
+    fft_line<T, M, true>(sre, sim, aux, t, get, put);
+    __syncthreads();
+    constexpr int NW = P > 64 ? P / 64 : 1;
+    half_post<T, M>(sre, sim, tot + li * 2 * NW, tot + li * 2 * NW + NW, t, scale);
+    for (int e = threadIdx.x; e < N * LPB; e += NT) {
+        int c = e % LPB, q = e / LPB;
+        if (c0 + c < ncols) {
+            const T *lre = smem + (2 * c) * LP, *lim = smem + (2 * c + 1) * LP;
+            out[base + c + q * elem_stride] = cconv<To, T>(mk<T>(lre[lds_pad(q + 1)], lim[lds_pad(q + 1)]));
+        }
+    }
+}
+
 // Dense DST: out[line][k] = sum_n S[k][n] in[line][n].  Block tile: 32 lines x
 // 64 outputs, 256 threads, each thread 8 outputs of one line.
 template <typename Ti, typename T, typename To>
@@ -203,9 +368,59 @@ static hipError_t launch_fft_L(const Lines &ln, const void *in, void *out, const
     return hipGetLastError();
 }
 
+template <typename Ti, typename T, typename To, int M>
+static hipError_t launch_half_M(const Lines &ln, const void *in, void *out, const void *aux, hipStream_t st) {
+    using cfg = half_cfg<M>;
+    constexpr int NW = cfg::P > 64 ? cfg::P / 64 : 1;
+    size_t shm = sizeof(T) * (2 * cfg::LPB * cfg::LP + 2 * cfg::LPB * NW);
+    T scale = (T)sqrt(2.0 / (double)M);
+    const C2<Ti> *pi = (const C2<Ti> *)in;
+    C2<To> *po = (C2<To> *)out;
+    const C2<T> *pa = (const C2<T> *)aux;
+    if (ln.elem_stride == 1 && ln.col_stride == ln.N && (ln.nplanes == 1 || ln.plane_stride == ln.ncols * ln.N)) {
+        long nrows = ln.ncols * ln.nplanes;
+        auto kfn = k_dst_half_rows<Ti, T, To, M>;
+        if (shm > 65536) {
+            hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+            if (e != hipSuccess) return e;
+        }
+        long grid = (nrows + cfg::LPB - 1) / cfg::LPB;
+        hipLaunchKernelGGL(kfn, dim3((unsigned)grid), dim3(cfg::THREADS), shm, st, pi, po, nrows, scale, pa);
+        return hipGetLastError();
+    }
+    if (ln.col_stride != 1) return hipErrorInvalidValue;
+    auto kfn = k_dst_half_cols<Ti, T, To, M>;
+    if (shm > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        if (e != hipSuccess) return e;
+    }
+    long ntiles = (ln.ncols + cfg::LPB - 1) / cfg::LPB;
+    long grid = ntiles * ln.nplanes;
+    hipLaunchKernelGGL(kfn, dim3((unsigned)grid), dim3(cfg::THREADS), shm, st, pi, po, ln.ncols, ntiles,
+                       ln.plane_stride, ln.elem_stride, scale, pa);
+    return hipGetLastError();
+}
+
 template <typename Ti, typename T, typename To>
-static hipError_t launch_dst_t(const Lines &ln, const void *in, void *out, const void *tw, const void *S,
+static hipError_t launch_dst_t(const Lines &ln, int algo, const void *in, void *out, const void *tw, const void *S,
                                hipStream_t st) {
+    if (fft_supported(ln.N) && algo == 0) {
+        switch (ln.N + 1) {
+        case 2: return launch_half_M<Ti, T, To, 2>(ln, in, out, tw, st);
+        case 4: return launch_half_M<Ti, T, To, 4>(ln, in, out, tw, st);
+        case 8: return launch_half_M<Ti, T, To, 8>(ln, in, out, tw, st);
+        case 16: return launch_half_M<Ti, T, To, 16>(ln, in, out, tw, st);
+        case 32: return launch_half_M<Ti, T, To, 32>(ln, in, out, tw, st);
+        case 64: return launch_half_M<Ti, T, To, 64>(ln, in, out, tw, st);
+        case 128: return launch_half_M<Ti, T, To, 128>(ln, in, out, tw, st);
+        case 256: return launch_half_M<Ti, T, To, 256>(ln, in, out, tw, st);
+        case 512: return launch_half_M<Ti, T, To, 512>(ln, in, out, tw, st);
+        case 1024: return launch_half_M<Ti, T, To, 1024>(ln, in, out, tw, st);
+        case 2048: return launch_half_M<Ti, T, To, 2048>(ln, in, out, tw, st);
+        case 4096: return launch_half_M<Ti, T, To, 4096>(ln, in, out, tw, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
     if (fft_supported(ln.N)) {
         switch (2 * (ln.N + 1)) {
         case 4: return launch_fft_L<Ti, T, To, 4>(ln, in, out, tw, st);
@@ -234,13 +449,14 @@ static hipError_t launch_dst_t(const Lines &ln, const void *in, void *out, const
 hipError_t launch_dst(const Lines &ln, DstTypes ty, const void *in, void *out, const void *tw, const void *S,
                       hipStream_t st) {
     if (ln.N < 1) return hipErrorInvalidValue;
+    const int a = ty.algo;
     if (ty.comp == 0) {
-        if (ty.in == 0 && ty.out == 0) return launch_dst_t<double, double, double>(ln, in, out, tw, S, st);
+        if (ty.in == 0 && ty.out == 0) return launch_dst_t<double, double, double>(ln, a, in, out, tw, S, st);
         return hipErrorInvalidValue;
     }
-    if (ty.in == 0 && ty.out == 1) return launch_dst_t<double, float, float>(ln, in, out, tw, S, st);
-    if (ty.in == 1 && ty.out == 1) return launch_dst_t<float, float, float>(ln, in, out, tw, S, st);
-    if (ty.in == 1 && ty.out == 0) return launch_dst_t<float, float, double>(ln, in, out, tw, S, st);
+    if (ty.in == 0 && ty.out == 1) return launch_dst_t<double, float, float>(ln, a, in, out, tw, S, st);
+    if (ty.in == 1 && ty.out == 1) return launch_dst_t<float, float, float>(ln, a, in, out, tw, S, st);
+    if (ty.in == 1 && ty.out == 0) return launch_dst_t<float, float, double>(ln, a, in, out, tw, S, st);
     return hipErrorInvalidValue;
 }
 

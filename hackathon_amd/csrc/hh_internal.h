@@ -8,8 +8,12 @@ namespace hh {
 enum Prec { F64 = 0, F32 = 1 };
 
 // Element types of a DST pass: input, compute, output (0 = double, 1 = float)
+// and, for power-of-two N+1, the transform: 0 = (N+1)-point FFT with
+// symmetric pre/post-processing (table: M twiddles then M sines), 1 =
+// 2(N+1)-point FFT of the odd extension (table: 2(N+1) twiddles).
 struct DstTypes {
     int in, comp, out;
+    int algo = 0;
 };
 
 // Lines of a DST-I pass.  Line l = (plane p, column c), p = l / ncols,

@@ -31,8 +31,14 @@ namespace hh {
 // dia = a lx ly + b lx + c ly + d.
 // With nop > 1 operators, right-hand side b uses the table at
 // coef + (b % nop) * cstride (one operator per problem of a batch).
-template <typename T> struct HelmBands {
-    int dim, n, Nx, nop;
+// One row of a tridiagonal system, returned by value (reference outputs
+// written under branches defeat scalar replacement and land in scratch).
+template <typename T> struct Band {
+    C2<T> sub, dia, sup;
+};
+
+template <typename T, int DIM> struct HelmBands {
+    int n, Nx, nop;
     const C2<T> *coef;
     long cstride;
     const double *lamx, *lamy;
@@ -40,7 +46,7 @@ template <typename T> struct HelmBands {
     HH_DEV Mode mode(long m, int bt) const {
         Mode md;
         md.cf = coef + (long)(bt % nop) * cstride;
-        if (dim == 2) {
+        if constexpr (DIM == 2) {
             md.l0 = (T)lamx[m]; md.l1 = 0; md.l2 = 0;
         } else {
             long i = m % Nx, j = m / Nx;
@@ -49,36 +55,39 @@ template <typename T> struct HelmBands {
         }
         return md;
     }
-    HH_DEV void band(const Mode &md, int q, C2<T> &sub, C2<T> &dia, C2<T> &sup) const {
-        if (dim == 2) {
+    HH_DEV Band<T> band(const Mode &md, int q) const {
+        Band<T> r;
+        const T ms = q > 0 ? (T)1 : (T)0, ps = q < n - 1 ? (T)1 : (T)0;
+        if constexpr (DIM == 2) {
             const C2<T> *c = md.cf + 6 * (long)q;
-            T l = md.l0;
-            sub = mk<T>(c[0].x * l + c[1].x, c[0].y * l + c[1].y);
-            dia = mk<T>(c[2].x * l + c[3].x, c[2].y * l + c[3].y);
-            sup = mk<T>(c[4].x * l + c[5].x, c[4].y * l + c[5].y);
+            const T l = md.l0;
+            r.sub = mk<T>(ms * (c[0].x * l + c[1].x), ms * (c[0].y * l + c[1].y));
+            r.dia = mk<T>(c[2].x * l + c[3].x, c[2].y * l + c[3].y);
+            r.sup = mk<T>(ps * (c[4].x * l + c[5].x), ps * (c[4].y * l + c[5].y));
         } else {
             const C2<T> *c = md.cf + 12 * (long)q;
-            T a = md.l0, b = md.l1, g = md.l2;
-            dia = mk<T>(c[0].x * a + c[1].x * b + c[2].x * g + c[3].x, c[0].y * a + c[1].y * b + c[2].y * g + c[3].y);
-            sup = mk<T>(c[4].x * a + c[5].x * b + c[6].x * g + c[7].x, c[4].y * a + c[5].y * b + c[6].y * g + c[7].y);
-            sub = mk<T>(c[8].x * a + c[9].x * b + c[10].x * g + c[11].x, c[8].y * a + c[9].y * b + c[10].y * g + c[11].y);
+            const T a = md.l0, b = md.l1, g = md.l2;
+            r.dia = mk<T>(c[0].x * a + c[1].x * b + c[2].x * g + c[3].x, c[0].y * a + c[1].y * b + c[2].y * g + c[3].y);
+            r.sup = mk<T>(ps * (c[4].x * a + c[5].x * b + c[6].x * g + c[7].x),
+                          ps * (c[4].y * a + c[5].y * b + c[6].y * g + c[7].y));
+            r.sub = mk<T>(ms * (c[8].x * a + c[9].x * b + c[10].x * g + c[11].x),
+                          ms * (c[8].y * a + c[9].y * b + c[10].y * g + c[11].y));
         }
-        if (q == 0) sub = mk<T>(0, 0);
-        if (q == n - 1) sup = mk<T>(0, 0);
+        return r;
     }
 };
 
 // Helmholtz problem: bands from HelmBands, right-hand side and solution in
 // the row-major field (element (b, m, q) at b*field + m + q*S).
-template <typename T, typename Ti, typename To> struct HelmProb {
-    HelmBands<T> bands;
+template <typename T, typename Ti, typename To, int DIM> struct HelmProb {
+    HelmBands<T, DIM> bands;
     const C2<Ti> *in;
     C2<To> *out;
     long field, S;
-    struct Mode { typename HelmBands<T>::Mode bm; long m; };
+    struct Mode { typename HelmBands<T, DIM>::Mode bm; long m; };
     HH_DEV int n() const { return bands.n; }
     HH_DEV Mode mode(long m, int bt) const { Mode md; md.bm = bands.mode(m, bt); md.m = m; return md; }
-    HH_DEV void band(const Mode &md, int q, C2<T> &a, C2<T> &b, C2<T> &c) const { bands.band(md.bm, q, a, b, c); }
+    HH_DEV Band<T> band(const Mode &md, int q) const { return bands.band(md.bm, q); }
     HH_DEV C2<T> rhs(int b, const Mode &md, int q) const { return cconv<T, Ti>(in[b * field + md.m + (long)q * S]); }
     HH_DEV void store(int b, const Mode &md, int q, C2<T> x) const {
         out[b * field + md.m + (long)q * S] = cconv<To, T>(x);
@@ -113,10 +122,12 @@ template <typename T> struct CompactProb {
         md.out = oorig + a * oA + b * oB;
         return md;
     }
-    HH_DEV void band(const Mode &, int q, C2<T> &a, C2<T> &b, C2<T> &c) const {
-        a = mk<T>(q > 0 ? (T)1 : (T)0, 0);
-        b = mk<T>(diag, 0);
-        c = mk<T>(q < nn - 1 ? (T)1 : (T)0, 0);
+    HH_DEV Band<T> band(const Mode &, int q) const {
+        Band<T> r;
+        r.sub = mk<T>(q > 0 ? (T)1 : (T)0, 0);
+        r.dia = mk<T>(diag, 0);
+        r.sup = mk<T>(q < nn - 1 ? (T)1 : (T)0, 0);
+        return r;
     }
     HH_DEV C2<T> rhs(int bt, const Mode &md, int q) const {
         const C2<T> *g = f + bt * field_in + md.in + (long)(q + 1) * eS;
@@ -160,7 +171,8 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
     C2<T> sub_s = mk<T>(0, 0), dia_s = mk<T>(1, 0), sup_s = mk<T>(0, 0), d_s = mk<T>(0, 0);
     C2<T> pa = mk<T>(0, 0), pc = mk<T>(0, 0), pd = mk<T>(0, 0);
     if (valid) {
-        prob.band(md, s, sub_s, dia_s, sup_s);
+        const Band<T> bs = prob.band(md, s);
+        sub_s = bs.sub; dia_s = bs.dia; sup_s = bs.sup;
         d_s = prob.rhs(bt, md, s);
     }
     // 1. downward sweep over rows s+1..e
@@ -170,7 +182,8 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
             const int q = s + 1 + t;
             C2<T> a = mk<T>(0, 0), b = mk<T>(1, 0), cc = mk<T>(0, 0), d = mk<T>(0, 0);
             if (valid) {
-                prob.band(md, q, a, b, cc);
+                const Band<T> bq = prob.band(md, q);
+                a = bq.sub; b = bq.dia; cc = bq.sup;
                 d = prob.rhs(bt, md, q);
             }
             C2<T> den = (t == 0) ? b : cfms<T>(a, pc, b);
@@ -282,17 +295,20 @@ static hipError_t launch_tri_prob(const Prob &pb, int n, long nmodes, int batch,
 }
 
 void tri_shape(int n, long nmodes, int prec, int *W, int *P, int *mmax) {
-    // rows per thread: 8 (FP64) keeps the three per-row register arrays
-    // within 128 VGPRs for 1024-thread blocks; FP32 affords 16.
-    int mm = prec == F64 ? (n > 1024 ? 16 : 8) : 16;
+    // Measured on MI355X (FP64, k = 50 compact operators): 256-thread blocks
+    // win at every length tried; 16 rows per thread above n = 512 (fewer
+    // chunks, a shallower reduced system), 8 below.  n = 1023, 64 x 1023
+    // modes: W = 4 x 16 rows 894 us against 1297 us for W = 4 x 8 rows;
+    // n = 255, 255^2 modes: W = 8 x 8 rows 209 us against 286 us for 16 x 8.
+    int mm = n > 512 ? 16 : 8;
     if (n <= 16) mm = 4;
+    (void)prec;
     int p = (n + mm - 1) / mm;
     if (p > n / 2) p = n / 2;
     if (p < 1) p = 1;
-    // modes per block: as many as fit the thread budget (512 threads for
-    // FP64 so that the chunk registers do not spill), at most 16
-    int w = (prec == F64 ? 512 : 1024) / p;
+    int w = 256 / p;
     if (w > 16) w = 16;
+    if (w < 1) w = 1;
     int pw = 1;
     while (pw * 2 <= w) pw *= 2;
     w = pw;
@@ -300,10 +316,9 @@ void tri_shape(int n, long nmodes, int prec, int *W, int *P, int *mmax) {
     *W = w; *P = p; *mmax = mm;
 }
 
-template <typename T, typename Ti, typename To, int MM>
-static hipError_t launch_tri_t(const TriHelm &t, const void *in, void *out, hipStream_t st) {
-    HelmProb<T, Ti, To> pb;
-    pb.bands.dim = t.dim;
+template <typename T, typename Ti, typename To, int MM, int DIM>
+static hipError_t launch_tri_d(const TriHelm &t, const void *in, void *out, hipStream_t st) {
+    HelmProb<T, Ti, To, DIM> pb;
     pb.bands.Nx = t.Nx;
     pb.bands.coef = (const C2<T> *)t.coef;
     pb.bands.nop = t.nop > 0 ? t.nop : 1;
@@ -317,6 +332,12 @@ static hipError_t launch_tri_t(const TriHelm &t, const void *in, void *out, hipS
     if (t.dim == 2) { pb.bands.n = t.Ny; nmodes = t.Nx; pb.S = t.Nx; }
     else { pb.bands.n = t.Nz; nmodes = (long)t.Nx * t.Ny; pb.S = (long)t.Nx * t.Ny; }
     return launch_tri_prob<T, MM>(pb, pb.bands.n, nmodes, t.batch, t.W, t.P, st);
+}
+
+template <typename T, typename Ti, typename To, int MM>
+static hipError_t launch_tri_t(const TriHelm &t, const void *in, void *out, hipStream_t st) {
+    if (t.dim == 2) return launch_tri_d<T, Ti, To, MM, 2>(t, in, out, st);
+    return launch_tri_d<T, Ti, To, MM, 3>(t, in, out, st);
 }
 
 template <typename T, typename Ti, typename To>

@@ -97,6 +97,10 @@ class Context:
     def synchronize(self):
         _check(lib().hh_context_synchronize(self.h))
 
+    def set_dst_algorithm(self, algo):
+        """0: (N+1)-point FFT with symmetric pre/post-pass; 1: 2(N+1)-point odd-extension FFT."""
+        _check(lib().hh_context_set_dst_algorithm(self.h, int(algo)))
+
     def set_memory_budget(self, nbytes):
         _check(lib().hh_context_set_memory_budget(self.h, ctypes.c_size_t(int(nbytes))))
 

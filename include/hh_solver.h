@@ -58,6 +58,10 @@ size_t hh_context_bytes_in_use(const hh_context *ctx);
  * milliseconds, the number of launches and the algorithmic bytes moved
  * (each element read once and written once). */
 int hh_context_enable_timing(hh_context *ctx, int on);
+/* DST-I of power-of-two N+1: 0 (default) = (N+1)-point FFT of the
+ * symmetrised row plus a prefix-sum post-pass; 1 = 2(N+1)-point FFT of the
+ * odd extension.  Both are orthonormal DST-I to roundoff. */
+int hh_context_set_dst_algorithm(hh_context *ctx, int algo);
 int hh_context_read_timing(hh_context *ctx, double ms[5], long launches[5], double bytes[5]);
 int hh_context_reset_timing(hh_context *ctx);
 
@@ -118,6 +122,13 @@ int hh_solve_host(hh_plan *plan, const double *h_rhs, double *h_sol, int batch);
 int hh_plan_set_refine(hh_plan *plan, double tol, int max_iter);
 /* sweeps and final relative residual of the last mixed solve */
 int hh_plan_last_refine(const hh_plan *plan, int *sweeps, double *rel_residual);
+/* Block shape of the tridiagonal kernel: W neighbouring modes per block,
+ * rows_per_thread in {4, 8, 16} (the chunk count follows).  Set by the
+ * autotuner; (0, 0) restores the built-in heuristic.  Shapes differ in
+ * speed and in roundoff only; for a given shape every right-hand side gives
+ * the same bits whatever else is in its batch. */
+int hh_plan_set_tri_shape(hh_plan *plan, int W, int rows_per_thread);
+int hh_plan_get_tri_shape(const hh_plan *plan, int *W, int *P, int *rows_per_thread);
 
 /* ---- residual: replaces residual(System) ----
  * 2D_Program/main_functions/residual.c:3-34 (9-point) and
