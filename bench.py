@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the oracle comparison (profiling runs)")
     ap.add_argument("--check-rhs", type=int, default=2, help="right-hand sides per order compared with the oracle")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB three times with an elementwise kernel (y = x * 1), the "
+                         "known-size reference for scaling HBM counters")
     return ap.parse_args()
 
 
@@ -225,6 +228,14 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.copy_probe > 0:
+        n = args.copy_probe * (1 << 20) // 8
+        x = torch.ones(n, dtype=torch.float64, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8}), file=sys.stderr)
 
 
 if __name__ == "__main__":
