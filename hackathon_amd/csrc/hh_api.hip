@@ -97,6 +97,8 @@ struct hh_plan {
     void *coef64 = nullptr, *coef32 = nullptr;   // [op][q][6] or [op][q][12]
     double *lamx = nullptr, *lamy = nullptr;
     std::vector<cd> hcoef;                       // host copy of the band table
+    std::vector<float> h32;                      // FP32 copy staged for hh_plan_set_operators
+    int cap_nop = 1;                             // operator slots allocated on the device
     double tol = 1e-12;
     int max_iter = 6;
     int tri_W = 0, tri_mm = 0;                   // tuned tridiagonal block shape (0 = heuristic)
@@ -508,6 +510,28 @@ extern "C" int hh_coefficients3d(int order, int Nx, int Ny, int Nz, double hx, d
 }
 
 // ------------------------------------------------------------------ plans
+// Device band-table order from the caller's operator vectors.
+// 2D: input [op][bp, ap, b, a, bm, am][Ny] -> table [op][q](bm, am, b, a, bp, ap);
+// 3D: input [op][a..dm][Nz] -> table [op][l][12].
+static void fill_table(hh_plan *p, int nop, const double *coef) {
+    if (p->dim == 2) {
+        const int Ny = p->Ny;
+        static const int src[6] = {4, 5, 2, 3, 0, 1};
+        p->hcoef.resize(6 * (size_t)Ny * nop);
+        for (int o = 0; o < nop; o++)
+            for (int q = 0; q < Ny; q++)
+                for (int t = 0; t < 6; t++)
+                    p->hcoef[((size_t)o * Ny + q) * 6 + t] = ld(coef, ((size_t)o * 6 + src[t]) * Ny + q);
+    } else {
+        const int Nz = p->Nz;
+        p->hcoef.resize(12 * (size_t)Nz * nop);
+        for (int o = 0; o < nop; o++)
+            for (int l = 0; l < Nz; l++)
+                for (int q = 0; q < 12; q++)
+                    p->hcoef[((size_t)o * Nz + l) * 12 + q] = ld(coef, ((size_t)o * 12 + q) * Nz + l);
+    }
+}
+
 static int upload_plan(hh_plan *p, int ncoef, int n) {
     hh_context *c = p->ctx;
     size_t b64 = (size_t)ncoef * n * p->nop * 16, b32 = (size_t)ncoef * n * p->nop * 8;
@@ -571,13 +595,8 @@ extern "C" int hh_plan2d_create_multi(hh_context *ctx, int Nx, int Ny, int nop, 
     if ((rc = set_device(ctx))) return rc;
     hh_plan *p = new hh_plan();
     p->ctx = ctx; p->dim = 2; p->Nx = Nx; p->Ny = Ny; p->Nz = 1; p->prec = precision; p->nop = nop;
-    p->hcoef.resize(6 * (size_t)Ny * nop);
-    // input [op][bp, ap, b, a, bm, am][Ny]; device band table [op][q](bm, am, b, a, bp, ap)
-    static const int src[6] = {4, 5, 2, 3, 0, 1};
-    for (int o = 0; o < nop; o++)
-        for (int q = 0; q < Ny; q++)
-            for (int t = 0; t < 6; t++)
-                p->hcoef[((size_t)o * Ny + q) * 6 + t] = ld(coef, ((size_t)o * 6 + src[t]) * Ny + q);
+    p->cap_nop = nop;
+    fill_table(p, nop, coef);
     if ((rc = upload_plan(p, 6, Ny))) { hh_plan_destroy(p); return rc; }
     *out = p;
     return HH_OK;
@@ -604,12 +623,8 @@ extern "C" int hh_plan3d_create_multi(hh_context *ctx, int Nx, int Ny, int Nz, i
     if ((rc = set_device(ctx))) return rc;
     hh_plan *p = new hh_plan();
     p->ctx = ctx; p->dim = 3; p->Nx = Nx; p->Ny = Ny; p->Nz = Nz; p->prec = precision; p->nop = nop;
-    p->hcoef.resize(12 * (size_t)Nz * nop);
-    // input [op][a..dm][Nz]; device table [op][l][12]
-    for (int o = 0; o < nop; o++)
-        for (int l = 0; l < Nz; l++)
-            for (int q = 0; q < 12; q++)
-                p->hcoef[((size_t)o * Nz + l) * 12 + q] = ld(coef, ((size_t)o * 12 + q) * Nz + l);
+    p->cap_nop = nop;
+    fill_table(p, nop, coef);
     if ((rc = upload_plan(p, 12, Nz))) { hh_plan_destroy(p); return rc; }
     *out = p;
     return HH_OK;
@@ -622,8 +637,8 @@ extern "C" int hh_plan_destroy(hh_plan *p) {
     hh_context *c = p->ctx;
     int nc = p->dim == 2 ? 6 : 12, n = p->dim == 2 ? p->Ny : p->Nz;
     (void)hipStreamSynchronize(c->stream);
-    dev_free(c, p->coef64, (size_t)nc * n * p->nop * 16);
-    dev_free(c, p->coef32, (size_t)nc * n * p->nop * 8);
+    dev_free(c, p->coef64, (size_t)nc * n * p->cap_nop * 16);
+    dev_free(c, p->coef32, (size_t)nc * n * p->cap_nop * 8);
     dev_free(c, p->lamx, (size_t)p->Nx * 8);
     if (p->lamy) dev_free(c, p->lamy, (size_t)p->Ny * 8);
     delete p;
@@ -787,11 +802,9 @@ static int norms_sync(hh_context *c, long n, const void *x, int x_f32, const voi
     return HH_OK;
 }
 
-extern "C" int hh_solve(hh_plan *p, const void *rhs, void *sol, int batch) {
-    if (!p || !rhs || !sol) return fail(HH_ERR_ARG, "hh_solve: null argument");
-    if (batch < 1) return fail(HH_ERR_ARG, "hh_solve: batch must be >= 1 (got %d)", batch);
+// hh_solve with the context lock held.
+static int solve_locked(hh_plan *p, const void *rhs, void *sol, int batch) {
     hh_context *c = p->ctx;
-    std::lock_guard<std::mutex> lk(c->mu);
     int rc = set_device(c);
     if (rc) return rc;
     size_t ws_bytes = hh_plan_workspace_bytes(p, batch);
@@ -807,6 +820,20 @@ extern "C" int hh_solve(hh_plan *p, const void *rhs, void *sol, int batch) {
     if (!plan_all_fft(p)) return fail(HH_ERR_UNSUPPORTED, "mixed precision needs power-of-two N+1");
     if (rhs == sol) return fail(HH_ERR_ARG, "mixed precision solve needs d_sol != d_rhs");
     if ((rc = solve_core(p, 0, 1, 0, 0, rhs, sol, dlt, batch))) return rc;   // u0 = A32^{-1} F
+    if (p->tol == 0) {
+        // fixed number of sweeps: no norms, no host synchronisation, so the
+        // whole solve can be recorded into a HIP graph
+        for (int it = 0; it < p->max_iter; it++) {
+            if ((rc = residual_into(p, sol, rhs, dlt, 1, batch))) return rc;
+            if ((rc = solve_core(p, 1, 1, 1, 0, dlt, dlt, dlt, batch))) return rc;
+            TimeScope ts(c, 3, (double)n * 40);
+            hipError_t e = hh::launch_axpy_update(n, sol, dlt, 1, -1.0, c->stream);
+            if (e != hipSuccess) return fail(HH_ERR_DEVICE, "update launch failed: %s", hipGetErrorString(e));
+        }
+        p->last_sweeps = p->max_iter;
+        p->last_rel = -1.0;
+        return HH_OK;
+    }
     double nf[2], nr[2];
     if ((rc = norms_sync(c, n, rhs, 0, nullptr, nf))) return rc;
     const double fn = nf[0] > 0 ? nf[0] : 1.0;
@@ -827,6 +854,144 @@ extern "C" int hh_solve(hh_plan *p, const void *rhs, void *sol, int batch) {
     }
     p->last_sweeps = sweeps;
     p->last_rel = rel;
+    return HH_OK;
+}
+
+extern "C" int hh_solve(hh_plan *p, const void *rhs, void *sol, int batch) {
+    if (!p || !rhs || !sol) return fail(HH_ERR_ARG, "hh_solve: null argument");
+    if (batch < 1) return fail(HH_ERR_ARG, "hh_solve: batch must be >= 1 (got %d)", batch);
+    std::lock_guard<std::mutex> lk(p->ctx->mu);
+    return solve_locked(p, rhs, sol, batch);
+}
+
+// ------------------------------------------------------------------ graphs
+struct hh_graph {
+    hh_context *ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int nodes = 0;
+};
+
+// Builds every table a plan's solves read (twiddles or DST matrices) so
+// that recording a solve needs no allocation or synchronous copy.
+static int prepare_tables(hh_plan *p) {
+    hh_context *c = p->ctx;
+    const int tc = p->prec == HH_PREC_F64 ? 0 : 1;
+    const int lens[2] = {p->Nx, p->dim == 3 ? p->Ny : 0};
+    for (int N : lens) {
+        if (N <= 0) continue;
+        void *t = nullptr;
+        int rc;
+        if (hh::fft_supported(N)) rc = c->dst_algo == 0 ? get_half_table(c, N, tc, &t) : get_twiddles(c, N, tc, &t);
+        else rc = get_dst_matrix(c, N, tc, &t);
+        if (rc) return rc;
+    }
+    return HH_OK;
+}
+
+extern "C" int hh_graph_create(hh_context *c, int nsolves, hh_plan *const *plans, void *const *d_rhs,
+                               void *const *d_sol, const int *batch, hh_graph **out) {
+    if (!c || !out || nsolves < 1 || !plans || !d_rhs || !d_sol || !batch)
+        return fail(HH_ERR_ARG, "hh_graph_create: bad arguments");
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = set_device(c);
+    if (rc) return rc;
+    size_t ws = 0;
+    for (int i = 0; i < nsolves; i++) {
+        hh_plan *p = plans[i];
+        if (!p || !d_rhs[i] || !d_sol[i] || batch[i] < 1) return fail(HH_ERR_ARG, "hh_graph_create: bad solve %d", i);
+        if (p->ctx != c) return fail(HH_ERR_ARG, "hh_graph_create: plan %d belongs to another context", i);
+        if (p->prec == HH_PREC_MIXED && p->tol != 0)
+            return fail(HH_ERR_UNSUPPORTED, "hh_graph_create: mixed-precision plan %d needs fixed sweeps "
+                                            "(hh_plan_set_refine(plan, 0, sweeps))", i);
+        if ((rc = prepare_tables(p))) return rc;
+        size_t b = hh_plan_workspace_bytes(p, batch[i]);
+        if (b > ws) ws = b;
+    }
+    if (ws && (rc = ensure_scratch(c, ws))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    hipStream_t cap = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    hipStream_t saved = c->stream;
+    const bool timing = c->timing;
+    c->stream = cap;
+    c->timing = false;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        for (int i = 0; i < nsolves && rc == HH_OK; i++) rc = solve_locked(plans[i], d_rhs[i], d_sol[i], batch[i]);
+        hipError_t e2 = hipStreamEndCapture(cap, &g);
+        if (rc == HH_OK && e2 != hipSuccess) rc = fail(HH_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e2));
+    } else {
+        rc = fail(HH_ERR_DEVICE, "hipStreamBeginCapture failed: %s", hipGetErrorString(e));
+    }
+    c->stream = saved;
+    c->timing = timing;
+    (void)hipStreamDestroy(cap);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(HH_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    hh_graph *G = new hh_graph();
+    G->ctx = c;
+    G->graph = g;
+    G->exec = x;
+    size_t nn = 0;
+    if (hipGraphGetNodes(g, nullptr, &nn) == hipSuccess) G->nodes = (int)nn;
+    *out = G;
+    return HH_OK;
+}
+
+extern "C" int hh_graph_launch(hh_graph *G) {
+    if (!G) return fail(HH_ERR_ARG, "hh_graph_launch: null graph");
+    hh_context *c = G->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = set_device(c);
+    if (rc) return rc;
+    HIPCHK(hipGraphLaunch(G->exec, c->stream));
+    return HH_OK;
+}
+
+extern "C" int hh_graph_num_nodes(const hh_graph *G) { return G ? G->nodes : 0; }
+
+extern "C" int hh_graph_destroy(hh_graph *G) {
+    if (!G) return HH_OK;
+    (void)hipStreamSynchronize(G->ctx->stream);
+    if (G->exec) (void)hipGraphExecDestroy(G->exec);
+    if (G->graph) (void)hipGraphDestroy(G->graph);
+    delete G;
+    return HH_OK;
+}
+
+extern "C" int hh_plan_set_operators(hh_plan *p, int nop, const double *coef) {
+    if (!p || !coef) return fail(HH_ERR_ARG, "hh_plan_set_operators: null argument");
+    if (nop < 1 || nop > p->cap_nop)
+        return fail(HH_ERR_ARG, "hh_plan_set_operators: %d operators, the plan holds at most %d", nop, p->cap_nop);
+    hh_context *c = p->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = set_device(c);
+    if (rc) return rc;
+    const int nc = p->dim == 2 ? 6 : 12, n = p->dim == 2 ? p->Ny : p->Nz;
+    // the host staging buffers feed the previous call's copies until they
+    // complete; wait for the stream before refilling them
+    HIPCHK(hipStreamSynchronize(c->stream));
+    fill_table(p, nop, coef);
+    const size_t cnt = (size_t)nc * n * nop;
+    p->h32.resize(2 * cnt);
+    for (size_t i = 0; i < cnt; i++) {
+        p->h32[2 * i] = (float)p->hcoef[i].real();
+        p->h32[2 * i + 1] = (float)p->hcoef[i].imag();
+    }
+    // stream-ordered: solves queued before this call still read the old tables
+    HIPCHK(hipMemcpyAsync(p->coef64, p->hcoef.data(), cnt * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(p->coef32, p->h32.data(), cnt * 8, hipMemcpyHostToDevice, c->stream));
+    p->nop = nop;
     return HH_OK;
 }
 

@@ -50,15 +50,61 @@ __global__ __launch_bounds__(dst_cfg<L>::THREADS) void k_dst_fft_rows(
     const bool valid = row < nrows;
     const C2<Ti> *x = in + (valid ? row : 0) * N;
     C2<To> *y = out + (valid ? row : 0) * N;
+    // Branch-free: every load is issued (index clamped into the row) and
+    // the zero entries of the odd extension are selected afterwards, so the
+    // 16 loads of a thread are all in flight together.
     auto get = [&](int n) -> C2<T> {
-        if (!valid || n == 0 || n == M) return mk<T>(0, 0);
-        if (n < M) return cconv<T, Ti>(x[n - 1]);
-        return cneg<T>(cconv<T, Ti>(x[L - n - 1]));
+        const bool zero = (n == 0) || (n == M);
+        const int i = zero ? 0 : (n < M ? n - 1 : L - n - 1);
+        const C2<T> v = cconv<T, Ti>(x[i]);
+        const C2<T> r = n < M ? v : cneg<T>(v);
+        return zero ? mk<T>(0, 0) : r;
     };
     auto put = [&](int k, C2<T> z) {
         if (valid && k >= 1 && k <= N) y[k - 1] = cconv<To, T>(mk<T>(-z.y * scale, z.x * scale));
     };
     fft_line<T, L, false>(sre, sim, tw, t, get, put);
+}
+
+// Coalesced load of a tile of LPB neighbouring strided lines into the LDS
+// line buffers (element q of line c at position q+1).  Every load of a
+// thread is issued before the first LDS write so that they overlap; lanes
+// past the end of the tile read the tile's first element and write nothing.
+template <typename Ti, typename T, int N, int LPB, int LP, int NT>
+HH_DEV void load_col_tile(T *smem, const C2<Ti> *in, long base, long c0, long ncols, long elem_stride) {
+    constexpr int ITER = (N * LPB + NT - 1) / NT;
+    C2<T> v[ITER];
+#pragma unroll
+    for (int i = 0; i < ITER; i++) {
+        const int e = threadIdx.x + i * NT;
+        const int c = e % LPB, q = e / LPB;
+        const bool ok = (e < N * LPB) && (c0 + c < ncols);
+        v[i] = cconv<T, Ti>(in[ok ? base + c + (long)q * elem_stride : base]);
+        if (!ok) v[i] = mk<T>(0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < ITER; i++) {
+        const int e = threadIdx.x + i * NT;
+        if (e < N * LPB) {
+            const int c = e % LPB, q = e / LPB;
+            smem[(2 * c) * LP + lds_pad(q + 1)] = v[i].x;
+            smem[(2 * c + 1) * LP + lds_pad(q + 1)] = v[i].y;
+        }
+    }
+}
+
+// Store of the tile back to its strided lines (positions 1..N of each line).
+template <typename T, typename To, int N, int LPB, int LP, int NT>
+HH_DEV void store_col_tile(const T *smem, C2<To> *out, long base, long c0, long ncols, long elem_stride) {
+    constexpr int ITER = (N * LPB + NT - 1) / NT;
+#pragma unroll
+    for (int i = 0; i < ITER; i++) {
+        const int e = threadIdx.x + i * NT;
+        const int c = e % LPB, q = e / LPB;
+        if (e < N * LPB && c0 + c < ncols)
+            out[base + c + (long)q * elem_stride] =
+                cconv<To, T>(mk<T>(smem[(2 * c) * LP + lds_pad(q + 1)], smem[(2 * c + 1) * LP + lds_pad(q + 1)]));
+    }
 }
 
 template <typename Ti, typename T, typename To, int L>
@@ -73,14 +119,7 @@ __global__ __launch_bounds__(dst_cfg<L>::THREADS) void k_dst_fft_cols(
     const long c0 = tile * LPB;
     const long base = plane * plane_stride + c0;
     // coalesced tile load: lanes run along the columns of one row
-    for (int e = threadIdx.x; e < N * LPB; e += NT) {
-        int c = e % LPB, q = e / LPB;
-        T *lre = smem + (2 * c) * LP, *lim = smem + (2 * c + 1) * LP;
-        C2<T> v = mk<T>(0, 0);
-        if (c0 + c < ncols) v = cconv<T, Ti>(in[base + c + q * elem_stride]);
-        lre[lds_pad(q + 1)] = v.x;
-        lim[lds_pad(q + 1)] = v.y;
-    }
+    load_col_tile<Ti, T, N, LPB, LP, NT>(smem, in, base, c0, ncols, elem_stride);
     __syncthreads();
     const int li = threadIdx.x / P, t = threadIdx.x % P;
     T *sre = smem + (2 * li) * LP, *sim = smem + (2 * li + 1) * LP;
@@ -97,13 +136,7 @@ __global__ __launch_bounds__(dst_cfg<L>::THREADS) void k_dst_fft_cols(
     };
     fft_line<T, L, true>(sre, sim, tw, t, get, put);
     __syncthreads();
-    for (int e = threadIdx.x; e < N * LPB; e += NT) {
-        int c = e % LPB, q = e / LPB;
-        if (c0 + c < ncols) {
-            const T *lre = smem + (2 * c) * LP, *lim = smem + (2 * c + 1) * LP;
-            out[base + c + q * elem_stride] = cconv<To, T>(mk<T>(lre[lds_pad(q + 1)], lim[lds_pad(q + 1)]));
-        }
-    }
+    store_col_tile<T, To, N, LPB, LP, NT>(smem, out, base, c0, ncols, elem_stride);
 }
 
 // ------------------------------------------------------------------------
@@ -199,11 +232,14 @@ __global__ __launch_bounds__(half_cfg<M>::THREADS) void k_dst_half_rows(const C2
     const bool valid = row < nrows;
     const C2<Ti> *x = in + (valid ? row : 0) * N;
     const T *sn = reinterpret_cast<const T *>(aux + M);
+    // branch-free (clamped indices, zero selected for n = 0) so that all
+    // loads of a thread are in flight together; rows past the end read row 0
     auto get = [&](int n) -> C2<T> {
-        if (!valid || n == 0) return mk<T>(0, 0);
-        const C2<T> a = cconv<T, Ti>(x[n - 1]), b = cconv<T, Ti>(x[M - n - 1]);
+        const int ia = n > 0 ? n - 1 : 0, ib = n > 0 ? M - n - 1 : 0;
+        const C2<T> a = cconv<T, Ti>(x[ia]), b = cconv<T, Ti>(x[ib]);
         const T s = sn[n];
-        return mk<T>(s * (a.x + b.x) + (T)0.5 * (a.x - b.x), s * (a.y + b.y) + (T)0.5 * (a.y - b.y));
+        const C2<T> r = mk<T>(s * (a.x + b.x) + (T)0.5 * (a.x - b.x), s * (a.y + b.y) + (T)0.5 * (a.y - b.y));
+        return n > 0 ? r : mk<T>(0, 0);
     };
     auto put = [&](int k, C2<T> z) {
         sre[lds_pad(k)] = z.x;
@@ -216,10 +252,14 @@ __global__ __launch_bounds__(half_cfg<M>::THREADS) void k_dst_half_rows(const C2
     // coalesced store of the block's rows (contiguous in memory)
     const long nr = nrows - row0 < LPB ? nrows - row0 : LPB;
     C2<To> *y = out + row0 * N;
-    for (int e = threadIdx.x; e < nr * N; e += NT) {
-        const int l = e / N, m = e % N + 1;
-        const T *lre = smem + (2 * l) * LP, *lim = smem + (2 * l + 1) * LP;
-        y[e] = cconv<To, T>(mk<T>(lre[lds_pad(m)], lim[lds_pad(m)]));
+    constexpr int ITER = (LPB * N + NT - 1) / NT;
+#pragma unroll
+    for (int i = 0; i < ITER; i++) {
+        const int e = threadIdx.x + i * NT;
+        if (e < nr * N) {
+            const int l = e / N, m = e % N + 1;
+            y[e] = cconv<To, T>(mk<T>(smem[(2 * l) * LP + lds_pad(m)], smem[(2 * l + 1) * LP + lds_pad(m)]));
+        }
     }
 }
 
@@ -234,14 +274,7 @@ __global__ __launch_bounds__(half_cfg<M>::THREADS) void k_dst_half_cols(const C2
     const long tile = blockIdx.x % ntiles, plane = blockIdx.x / ntiles;
     const long c0 = tile * LPB;
     const long base = plane * plane_stride + c0;
-    for (int e = threadIdx.x; e < N * LPB; e += NT) {
-        int c = e % LPB, q = e / LPB;
-        T *lre = smem + (2 * c) * LP, *lim = smem + (2 * c + 1) * LP;
-        C2<T> v = mk<T>(0, 0);
-        if (c0 + c < ncols) v = cconv<T, Ti>(in[base + c + q * elem_stride]);
-        lre[lds_pad(q + 1)] = v.x;
-        lim[lds_pad(q + 1)] = v.y;
-    }
+    load_col_tile<Ti, T, N, LPB, LP, NT>(smem, in, base, c0, ncols, elem_stride);
     __syncthreads();
     const int li = threadIdx.x / P, t = threadIdx.x % P;
     T *sre = smem + (2 * li) * LP, *sim = smem + (2 * li + 1) * LP;
@@ -262,13 +295,7 @@ __global__ __launch_bounds__(half_cfg<M>::THREADS) void k_dst_half_cols(const C2
     __syncthreads();
     constexpr int NW = P > 64 ? P / 64 : 1;
     half_post<T, M>(sre, sim, tot + li * 2 * NW, tot + li * 2 * NW + NW, t, scale);
-    for (int e = threadIdx.x; e < N * LPB; e += NT) {
-        int c = e % LPB, q = e / LPB;
-        if (c0 + c < ncols) {
-            const T *lre = smem + (2 * c) * LP, *lim = smem + (2 * c + 1) * LP;
-            out[base + c + q * elem_stride] = cconv<To, T>(mk<T>(lre[lds_pad(q + 1)], lim[lds_pad(q + 1)]));
-        }
-    }
+    store_col_tile<T, To, N, LPB, LP, NT>(smem, out, base, c0, ncols, elem_stride);
 }
 
 // Dense DST: out[line][k] = sum_n S[k][n] in[line][n].  Block tile: 32 lines x

@@ -163,56 +163,61 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
     const long m = xcd_tile(blockIdx.x, gridDim.x) * W + w;
     const bool valid = (c < P) && (m < nmodes);
     const int bt = blockIdx.y;
-    const int s = (int)(((long)c * n) / P), e = (int)(((long)(c + 1) * n) / P) - 1;
-    const int len = e - s + 1;
-    const auto md = prob.mode(valid ? m : 0, bt);
+    // Idle lanes (c >= P, padding modes) run the same code on a clamped
+    // chunk and mode so that every load is in bounds and no branch splits
+    // the sweeps; their results are discarded.
+    const int cq = c < P ? c : P - 1;
+    const int s = (int)(((long)cq * n) / P), e = (int)(((long)(cq + 1) * n) / P) - 1;
+    const int len = e - s + 1;   // 2 <= len <= MMAX
+    const auto md = prob.mode(m < nmodes ? m : nmodes - 1, bt);
 
-    C2<T> ab[MMAX - 1], cb[MMAX - 1], db[MMAX - 1];
-    C2<T> sub_s = mk<T>(0, 0), dia_s = mk<T>(1, 0), sup_s = mk<T>(0, 0), d_s = mk<T>(0, 0);
-    C2<T> pa = mk<T>(0, 0), pc = mk<T>(0, 0), pd = mk<T>(0, 0);
-    if (valid) {
-        const Band<T> bs = prob.band(md, s);
-        sub_s = bs.sub; dia_s = bs.dia; sup_s = bs.sup;
-        d_s = prob.rhs(bt, md, s);
-    }
-    // 1. downward sweep over rows s+1..e
+    // 0. issue every right-hand-side load of the chunk up front (rows past
+    //    e repeat row e), so that the HBM latency is paid once per chunk and
+    //    not once per row of the recurrence
+    C2<T> db[MMAX], ab[MMAX], cb[MMAX];
 #pragma unroll
-    for (int t = 0; t < MMAX - 1; t++) {
-        if (t < len - 1) {
-            const int q = s + 1 + t;
-            C2<T> a = mk<T>(0, 0), b = mk<T>(1, 0), cc = mk<T>(0, 0), d = mk<T>(0, 0);
-            if (valid) {
-                const Band<T> bq = prob.band(md, q);
-                a = bq.sub; b = bq.dia; cc = bq.sup;
-                d = prob.rhs(bt, md, q);
-            }
-            C2<T> den = (t == 0) ? b : cfms<T>(a, pc, b);
-            C2<T> r = crcp<T>(den);
-            C2<T> na = (t == 0) ? cmul<T>(a, r) : cneg<T>(cmul<T>(cmul<T>(a, pa), r));
-            C2<T> nd = (t == 0) ? cmul<T>(d, r) : cmul<T>(cfms<T>(a, pd, d), r);
-            C2<T> nc = cmul<T>(cc, r);
+    for (int t = 0; t < MMAX; t++) db[t] = prob.rhs(bt, md, min(s + t, e));
+    const Band<T> bs = prob.band(md, s);
+    const C2<T> sub_s = bs.sub, dia_s = bs.dia, sup_s = bs.sup, d_s = db[0];
+
+    // 1. downward sweep over rows s+1..e: row s+t becomes
+    //    ab[t] x_s + x_{s+t} + cb[t] x_{s+t+1} = db[t]
+    C2<T> pa = mk<T>(0, 0), pc = mk<T>(0, 0), pd = mk<T>(0, 0);
+    Band<T> bn = prob.band(md, min(s + 1, e));
+#pragma unroll
+    for (int t = 1; t < MMAX; t++) {
+        // the next row's band is fetched one row ahead of its use
+        const Band<T> bq = bn;
+        if (t + 1 < MMAX) bn = prob.band(md, min(s + t + 1, e));
+        const C2<T> a = bq.sub, d = db[t];
+        const C2<T> den = (t == 1) ? bq.dia : cfms<T>(a, pc, bq.dia);
+        const C2<T> r = crcp<T>(den);
+        const C2<T> na = (t == 1) ? cmul<T>(a, r) : cneg<T>(cmul<T>(cmul<T>(a, pa), r));
+        const C2<T> nd = (t == 1) ? cmul<T>(d, r) : cmul<T>(cfms<T>(a, pd, d), r);
+        const C2<T> nc = cmul<T>(bq.sup, r);
+        if (t < len) {   // selects, not a branch
             ab[t] = na; cb[t] = nc; db[t] = nd;
             pa = na; pc = nc; pd = nd;
         }
     }
     // reduced row 2c+1 (x_e): pa x_s + x_e + pc x_{e+1} = pd  (already normalised)
     C2<T> A1 = pa, C1 = pc, D1 = pd;
-    // 2. upward sweep over rows e-2..s+1 (row e-1 keeps its downward form)
+    // 2. upward sweep over rows e-2..s+1 (row e-1 keeps its downward form):
+    //    row s+t becomes ab[t] x_s + x_{s+t} + cb[t] x_e = db[t]
 #pragma unroll
-    for (int t = MMAX - 3; t >= 0; t--) {
-        if (t <= len - 4) {
-            C2<T> cbt = cb[t];
-            ab[t] = cfms<T>(cbt, ab[t + 1], ab[t]);
-            db[t] = cfms<T>(cbt, db[t + 1], db[t]);
-            cb[t] = cneg<T>(cmul<T>(cbt, cb[t + 1]));
-        }
+    for (int t = MMAX - 3; t >= 1; t--) {
+        const C2<T> cbt = cb[t];
+        const C2<T> na = cfms<T>(cbt, ab[t + 1], ab[t]);
+        const C2<T> nd = cfms<T>(cbt, db[t + 1], db[t]);
+        const C2<T> nc = cneg<T>(cmul<T>(cbt, cb[t + 1]));
+        if (t <= len - 3) { ab[t] = na; db[t] = nd; cb[t] = nc; }
     }
     // reduced row 2c (x_s): sub_s x_{s-1} + B x_s + C x_e = D
     C2<T> B0, C0, D0;
     if (len >= 3) {
-        B0 = cfms<T>(sup_s, ab[0], dia_s);
-        C0 = cneg<T>(cmul<T>(sup_s, cb[0]));
-        D0 = cfms<T>(sup_s, db[0], d_s);
+        B0 = cfms<T>(sup_s, ab[1], dia_s);
+        C0 = cneg<T>(cmul<T>(sup_s, cb[1]));
+        D0 = cfms<T>(sup_s, db[1], d_s);
     } else {
         B0 = dia_s; C0 = sup_s; D0 = d_s;
     }
@@ -267,10 +272,10 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
     // 4. back-substitution of the interior and store
     prob.store(bt, md, s, xs);
 #pragma unroll
-    for (int t = 0; t < MMAX - 2; t++) {
-        if (t < len - 2) {
+    for (int t = 1; t < MMAX - 1; t++) {
+        if (t < len - 1) {
             C2<T> x = csub<T>(db[t], cadd<T>(cmul<T>(ab[t], xs), cmul<T>(cb[t], xe)));
-            prob.store(bt, md, s + 1 + t, x);
+            prob.store(bt, md, s + t, x);
         }
     }
     prob.store(bt, md, e, xe);
@@ -284,7 +289,10 @@ static hipError_t launch_tri_prob(const Prob &pb, int n, long nmodes, int batch,
     int threads = ((W * P + 63) / 64) * 64;
     size_t shm = sizeof(C2<T>) * 3 * 2 * (size_t)P * W;
     // 512-thread bound: 256 VGPRs per lane, no spills of the chunk registers
+    // up to 8 FP64 rows per thread; 16 FP64 rows need the 256-thread bound
+    // (one wave per SIMD, the whole register file) to stay out of scratch
     auto kfn = threads <= 512 ? k_tri<T, MM, 512, Prob> : k_tri<T, MM, 1024, Prob>;
+    if (sizeof(T) == 8 && MM >= 16 && threads <= 256) kfn = k_tri<T, MM, 256, Prob>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;

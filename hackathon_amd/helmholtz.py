@@ -244,6 +244,15 @@ class Plan:
         _check(lib().hh_plan_last_refine(self.h, ctypes.byref(s), ctypes.byref(r)))
         return s.value, r.value
 
+    def set_operators(self, coef):
+        """Replace the operators (at most as many as the plan was created with)."""
+        nvec, n = (6, self.Ny) if self.dim == 2 else (12, self.Nz)
+        ops = _operators(coef, nvec, n)
+        packed = np.ascontiguousarray(np.stack([np.stack(o) for o in ops]))
+        self.ctx.bind_stream()
+        _check(lib().hh_plan_set_operators(self.h, len(ops), _hptr(packed)))
+        self.nop, self.ops, self.coef = len(ops), ops, ops[0]
+
     def close(self):
         if self.h:
             lib().hh_plan_destroy(self.h)
@@ -326,3 +335,108 @@ class Plan3D(Plan):
                               _dptr(f_e), _dptr(u_e), dp(fxx_f), dp(fyy_f), dp(fzz_f), _hptr(k2e), _hptr(k2d),
                               _dptr(out), batch))
         return out
+
+
+class Graph:
+    """A fixed list of solves [(plan, rhs, out), ...] recorded into one HIP
+    graph; launch() replays all of them with one call on the context's stream.
+    The tensors must stay alive (and at the same addresses) while the graph
+    exists."""
+
+    def __init__(self, ctx, solves):
+        self.ctx = ctx
+        self.h = None
+        n = len(solves)
+        plans = (ctypes.c_void_p * n)(*[p.h.value for p, _, _ in solves])
+        rhs = (ctypes.c_void_p * n)(*[r.data_ptr() for _, r, _ in solves])
+        out = (ctypes.c_void_p * n)(*[o.data_ptr() for _, _, o in solves])
+        batch = (ctypes.c_int * n)(*[p._batch_of(r) for p, r, _ in solves])
+        self._keep = solves
+        ctx.bind_stream()
+        h = ctypes.c_void_p()
+        _check(lib().hh_graph_create(ctx.h, n, plans, rhs, out, batch, ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def num_nodes(self):
+        return int(lib().hh_graph_num_nodes(self.h))
+
+    def launch(self):
+        self.ctx.bind_stream()
+        _check(lib().hh_graph_launch(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hh_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ serving
+class _Request(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int), ("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("Nz", ctypes.c_int),
+                ("order", ctypes.c_int), ("precision", ctypes.c_int), ("k_re", ctypes.c_double),
+                ("k_im", ctypes.c_double), ("hx", ctypes.c_double), ("hy", ctypes.c_double),
+                ("hz", ctypes.c_double), ("h_rhs", ctypes.c_void_p), ("h_sol", ctypes.c_void_p)]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("queue_s", ctypes.c_double), ("latency_s", ctypes.c_double),
+                ("sweeps", ctypes.c_int), ("rel_residual", ctypes.c_double)]
+
+
+class Server:
+    """Continuous-batching request loop of libhh.so (hh_server_*).
+
+    submit() takes a host right-hand side (complex128, Nx*Ny(*Nz) values) and
+    returns a ticket; wait(ticket) blocks until the solution is in host memory
+    and returns (solution, info) with the queueing time and the latency from
+    arrival to result."""
+
+    def __init__(self, device=0, hbm_budget=0, lanes=2, max_batch=8):
+        h = ctypes.c_void_p()
+        _check(lib().hh_server_create(device, ctypes.c_size_t(int(hbm_budget)), lanes, max_batch, ctypes.byref(h)))
+        self.h = h
+        self._live = {}
+
+    def submit(self, dim, N, order, k, rhs, precision=PREC_F64, h=None):
+        Nx, Ny, Nz = (N, N, N) if np.isscalar(N) else (tuple(N) + (1,))[:3]
+        rhs = _c128(rhs)
+        sol = np.empty_like(rhs)
+        hx, hy, hz = (1.0 / (Nx + 1), 1.0 / (Ny + 1), 1.0 / (Nz + 1)) if h is None else h
+        k = complex(k)
+        r = _Request(dim, Nx, Ny, Nz if dim == 3 else 1, order, precision, k.real, k.imag, hx, hy, hz,
+                     rhs.ctypes.data, sol.ctypes.data)
+        t = ctypes.c_long()
+        _check(lib().hh_server_submit(self.h, ctypes.byref(r), ctypes.byref(t)))
+        self._live[t.value] = (rhs, sol)
+        return t.value
+
+    def wait(self, ticket):
+        res = _Result()
+        rc = lib().hh_server_wait(self.h, ctypes.c_long(ticket), ctypes.byref(res))
+        rhs, sol = self._live.pop(ticket)
+        _check(rc)
+        return sol, {"queue_s": res.queue_s, "latency_s": res.latency_s, "sweeps": res.sweeps,
+                     "rel_residual": res.rel_residual}
+
+    def stats(self):
+        b, s, r = ctypes.c_long(), ctypes.c_long(), ctypes.c_size_t()
+        _check(lib().hh_server_stats(self.h, ctypes.byref(b), ctypes.byref(s), ctypes.byref(r)))
+        return {"batches": b.value, "solved": s.value, "reserved_bytes": r.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hh_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -119,9 +119,19 @@ size_t hh_plan_workspace_bytes(const hh_plan *plan, int batch);
  * residual until ||r||/||rhs|| <= tol or max_iter sweeps (hh_plan_set_refine). */
 int hh_solve(hh_plan *plan, const void *d_rhs, void *d_sol, int batch);
 int hh_solve_host(hh_plan *plan, const double *h_rhs, double *h_sol, int batch);
+/* tol > 0: refine until ||r||inf/||rhs||inf <= tol or max_iter sweeps (each
+ * check reads two norms back to the host).  tol = 0: exactly max_iter sweeps
+ * with no norm and no host synchronisation, the form a HIP graph can hold. */
 int hh_plan_set_refine(hh_plan *plan, double tol, int max_iter);
-/* sweeps and final relative residual of the last mixed solve */
+/* sweeps and final relative residual of the last mixed solve (-1 when the
+ * sweeps were fixed and no residual was measured) */
 int hh_plan_last_refine(const hh_plan *plan, int *sweeps, double *rel_residual);
+/* Replace the operators of a plan made by hh_plan*_create_multi, with
+ * 1 <= nop <= the count it was created with (no reference counterpart: lets
+ * a serving loop reuse one device-resident plan per grid for every batch).
+ * coef has the layout of the create_multi call.  Waits for the work already
+ * queued on the context's stream, then queues the table copies on it. */
+int hh_plan_set_operators(hh_plan *plan, int nop, const double *coef);
 /* Block shape of the tridiagonal kernel: W neighbouring modes per block,
  * rows_per_thread in {4, 8, 16} (the chunk count follows).  Set by the
  * autotuner; (0, 0) restores the built-in heuristic.  Shapes differ in
@@ -129,6 +139,22 @@ int hh_plan_last_refine(const hh_plan *plan, int *sweeps, double *rel_residual);
  * the same bits whatever else is in its batch. */
 int hh_plan_set_tri_shape(hh_plan *plan, int W, int rows_per_thread);
 int hh_plan_get_tri_shape(const hh_plan *plan, int *W, int *P, int *rows_per_thread);
+
+/* ---- graphs: a fixed list of solves replayed with one launch ----
+ * No reference counterpart (the reference solves one problem per call): for
+ * batches of uneven problems, e.g. one plan per grid size.  Solve i runs
+ * plans[i] on batch[i] right-hand sides d_rhs[i] -> d_sol[i].  All plans
+ * belong to ctx; mixed-precision plans must use fixed sweeps
+ * (hh_plan_set_refine(plan, 0, n)).  Tables and scratch are built at
+ * creation, so a launch allocates nothing; the buffers must stay valid and
+ * the solves of one graph share the context's scratch in sequence. */
+typedef struct hh_graph hh_graph;
+int hh_graph_create(hh_context *ctx, int nsolves, hh_plan *const *plans, void *const *d_rhs, void *const *d_sol,
+                    const int *batch, hh_graph **graph);
+/* replay on the context's stream (asynchronous) */
+int hh_graph_launch(hh_graph *graph);
+int hh_graph_num_nodes(const hh_graph *graph);
+int hh_graph_destroy(hh_graph *graph);
 
 /* ---- residual: replaces residual(System) ----
  * 2D_Program/main_functions/residual.c:3-34 (9-point) and
@@ -180,6 +206,44 @@ int hh_rhs3d_host(hh_plan *plan, int order, double hx, double hy, double hz, con
 /* ---- norms: normInf of 2D_Program/helpers_functions/norms.c:3-13 ----
  * out[0] = max |x|, out[1] = max |x - y| (0 when d_y is NULL); blocks. */
 int hh_norms(hh_context *ctx, long n, const void *d_x, const void *d_y, double out[2]);
+
+/* ---- serving: continuous batching of mixed 2D/3D requests ----
+ * No reference counterpart (the reference's main() solves one problem per
+ * process).  A server runs `lanes` worker threads, each with its own context
+ * and stream; an idle lane takes the oldest waiting request together with
+ * the other waiting requests of the same grid and precision (at most
+ * max_batch), if the batch fits in the HBM budget (0 = unlimited), and
+ * solves them as one batched multi-operator solve on its resident plan for
+ * that grid.  Requests arriving meanwhile join later batches.
+ * One request: constant wavenumber k on the grid with spacings h*, compact
+ * scheme of the given order, one right-hand side in host memory (complex
+ * interleaved, row-major); the solution is written to h_sol.  Both buffers
+ * must stay valid until hh_server_wait returns for the ticket. */
+typedef struct hh_server hh_server;
+typedef struct {
+    int dim, Nx, Ny, Nz;      /* Nz ignored in 2D */
+    int order;                /* 2, 4 or 6 */
+    int precision;            /* HH_PREC_F64 or HH_PREC_MIXED */
+    double k_re, k_im;
+    double hx, hy, hz;
+    const double *h_rhs;
+    double *h_sol;
+} hh_request;
+typedef struct {
+    int status;               /* HH_OK or the solve's error */
+    double queue_s;           /* arrival to start of its batch */
+    double latency_s;         /* arrival to solution in host memory */
+    int sweeps;               /* refinement sweeps (mixed precision) */
+    double rel_residual;      /* final relative residual of its batch (mixed) */
+} hh_result;
+int hh_server_create(int device, size_t hbm_budget, int lanes, int max_batch, hh_server **server);
+/* non-blocking; *ticket identifies the request for hh_server_wait */
+int hh_server_submit(hh_server *server, const hh_request *req, long *ticket);
+/* blocks until the request is solved; returns its status (once per ticket) */
+int hh_server_wait(hh_server *server, long ticket, hh_result *result);
+int hh_server_stats(hh_server *server, long *batches, long *solved, size_t *reserved_bytes);
+/* finishes every submitted request, then stops the lanes */
+int hh_server_destroy(hh_server *server);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,92 @@
+"""GPU tests of the serving loop (hh_server_*): continuous batching of mixed
+2D/3D requests on several lanes.
+
+Bars: an FP64 request gives the same bits alone and in a mix of other
+requests (other grids, orders, wavenumbers, precisions, lanes); every result
+matches the FP64 oracle (FP64 <= 1e-12, mixed precision <= 1e-10); a
+request larger than the HBM budget is refused with HH_ERR_NOMEM.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def H():
+    import torch
+    assert torch.cuda.is_available(), "GPU test run without a GPU"
+    from hackathon_amd import helmholtz
+    return helmholtz
+
+
+def relerr(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def problem(oracle, dim, order, N, k, modes):
+    if dim == 2:
+        return oracle.solve2d(order, N, 1, [k, 0.0, modes[0], modes[1], 0.7], threads=8)
+    return oracle.solve3d(order, N, 1, [k, 0.0, modes[0], modes[1], modes[2], 1.0, 1.0], threads=8)
+
+
+def test_serving_alone_equals_mix(H, oracle):
+    target = (2, 6, 127, 30.0, (3, 2))
+    mix = [(2, 4, 63, 12.0, (1, 4)), (3, 6, 31, 9.0, (1, 2, 1)), (2, 6, 255, 55.0, (5, 7)), target,
+           (2, 4, 127, 41.0, (2, 2)), (3, 4, 15, 5.0, (1, 1, 2)), (2, 6, 127, 17.0, (4, 1)), target,
+           (2, 4, 63, 25.0, (6, 3)), (3, 6, 31, 14.0, (2, 1, 1))]
+    probs = {}
+    for c in mix:
+        if c not in probs:
+            probs[c] = problem(oracle, *c)
+    srv = H.Server(lanes=3, max_batch=4)
+    dim, order, N, k, _ = target
+    s = probs[target]
+    say("target alone")
+    alone, info = srv.wait(srv.submit(dim, N, order, k, s["rhs"]))
+    assert relerr(alone.ravel(), s["sol"]) <= 1e-12
+    say(f"mix of {len(mix)} requests (+2 mixed precision)")
+    tickets = [(c, srv.submit(c[0], c[2], c[1], c[3], probs[c]["rhs"])) for c in mix]
+    mixed_c = mix[2]
+    tm = [srv.submit(mixed_c[0], mixed_c[2], mixed_c[1], mixed_c[3], probs[mixed_c]["rhs"], precision=H.PREC_MIXED)
+          for _ in range(2)]
+    lat = []
+    for c, t in tickets:
+        sol, info = srv.wait(t)
+        lat.append(info["latency_s"])
+        err = relerr(sol.ravel(), probs[c]["sol"])
+        assert err <= 1e-12, (c, err)
+        if c == target:
+            assert np.array_equal(sol, alone), "target differs between alone and in the mix"
+    for t in tm:
+        sol, info = srv.wait(t)
+        err = relerr(sol.ravel(), probs[mixed_c]["sol"])
+        say(f"  mixed request sweeps={info['sweeps']} rel={info['rel_residual']:.1e} err={err:.2e}")
+        assert err <= 1e-10
+    st = srv.stats()
+    say(f"  {st['solved']} solved in {st['batches']} batches, max latency {max(lat) * 1e3:.1f} ms")
+    assert st["solved"] == len(mix) + 3
+    srv.close()
+
+
+def test_serving_budget_refuses_oversized(H):
+    srv = H.Server(hbm_budget=1 << 20, lanes=1, max_batch=2)   # 1 MiB
+    N = 255                                                    # 1 MiB per field
+    rhs = np.ones(N * N, np.complex128)
+    say("oversized request against a 1 MiB budget")
+    t = srv.submit(2, N, 4, 10.0, rhs)
+    with pytest.raises(H.HHError) as e:
+        srv.wait(t)
+    assert e.value.status == H.HH_ERR_NOMEM
+    # a request that fits still runs
+    small = np.ones(31 * 31, np.complex128)
+    sol, _ = srv.wait(srv.submit(2, 31, 4, 10.0, small))
+    assert np.isfinite(sol).all()
+    srv.close()
