@@ -47,6 +47,14 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the oracle comparison (profiling runs)")
     ap.add_argument("--check-rhs", type=int, default=2, help="right-hand sides per order compared with the oracle")
+    ap.add_argument("--workload", default="c2", choices=["c2", "3d", "uneven", "serve"],
+                    help="c2: the headline 2D batch (default); 3d: 255^3 FP64 and mixed precision; uneven: 512 "
+                         "uneven 2D problems in one HIP graph; serve: Poisson requests through the server")
+    ap.add_argument("--qps", type=float, default=100.0, help="serve: mean request rate")
+    ap.add_argument("--requests", type=int, default=300, help="serve: requests in the timed stream")
+    ap.add_argument("--lanes", type=int, default=4, help="serve: server lanes (streams)")
+    ap.add_argument("--problems", type=int, default=512, help="uneven: number of problems")
+    ap.add_argument("--sweeps", type=int, default=2, help="uneven: fixed refinement sweeps")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB three times with an elementwise kernel (y = x * 1), the "
                          "known-size reference for scaling HBM counters")
@@ -100,6 +108,310 @@ def cpu_baseline(O, N, coef, F_host, seconds):
                       f"{done} 6th-order right-hand sides of the same N={N} k=50 workload, {t_used:.1f} s"}
 
 
+# ---------------------------------------------------------------- other workloads
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _dof_line(metric_extra, value, unit, steps, warmup, ms, config, **kw):
+    out = {"metric": "complex DOF/s at <=1e-10 rel err vs FP64 oracle; p50/p99 solve latency" + metric_extra,
+           "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": kw.pop("dtype", "f64"),
+           "data": "synthetic", "config": config}
+    out.update(kw)
+    return out
+
+
+def _timing(H, ctx):
+    import ctypes
+    lib = H.lib()
+    ms = (ctypes.c_double * 5)()
+    nl = (ctypes.c_long * 5)()
+    by = (ctypes.c_double * 5)()
+    lib.hh_context_read_timing(ctx.h, ms, nl, by)
+    return {CATS[c]: {"ms": round(ms[c], 4), "launches": int(nl[c]),
+                      "avg_us": round(1e3 * ms[c] / nl[c], 2) if nl[c] else None,
+                      "GBps": round(by[c] / (ms[c] * 1e-3) / 1e9, 1) if ms[c] > 0 else None}
+            for c in range(5) if nl[c]}
+
+
+def manufactured3d(torch, N, kk, abc, dev, amp=1 + 1j):
+    """hz^2 f and u on the extended grid for u = amp sin(a pi x) sin(b pi y) sin(c pi z)."""
+    h = 1.0 / (N + 1)
+    x = torch.arange(N + 2, dtype=torch.float64, device=dev) * h
+    a, b, c = abc
+    sx, sy, sz = (torch.sin(m * math.pi * x) for m in (a, b, c))
+    u = (sz[:, None, None] * sy[None, :, None] * sx[None, None, :]).to(torch.complex128) * amp
+    fac = kk * kk - math.pi ** 2 * (a * a + b * b + c * c)
+    return (h * h * fac) * u, u
+
+
+def run_3d(args, torch, H):
+    """3D order 6, N^3 = 255^3, k = 30: plain FP64 against FP32 bulk + FP64 refinement."""
+    dev = torch.device("cuda:0")
+    N, kk, order = 255 if args.N == 1023 else args.N, 30.0, 6
+    h = 1.0 / (N + 1)
+    rng = np.random.default_rng(args.seed)
+    abc = [int(v) for v in rng.integers(1, 21, size=3)]
+    ctx = H.Context(0)
+    k2e = np.full(N + 2, kk * kk, np.complex128)
+    coef = H.coefficients3d(order, N, N, N, h, h, h, k2e)
+    p64 = H.Plan3D(ctx, N, N, N, coef)
+    pmx = H.Plan3D(ctx, N, N, N, coef, precision=H.PREC_MIXED)
+    pmx.set_refine(1e-12, 6)
+    f_e, u_e = manufactured3d(torch, N, kk, abc, dev)
+    zf = torch.zeros(2 * (N + 2) * (N + 2), dtype=torch.complex128, device=dev)
+    F = p64.rhs(order, h, h, h, f_e, u_e, k2e, fxx_f=zf, fyy_f=zf, fzz_f=zf)
+    u_exact = u_e[1:-1, 1:-1, 1:-1].contiguous()
+    del f_e, u_e
+    s64, smx = torch.empty_like(F), torch.empty_like(F)
+    res = {}
+    log(f"3d: N={N} (a,b,c)={abc} right-hand side assembled")
+    for name, plan, out in (("fp64", p64, s64), ("mixed", pmx, smx)):
+        log(f"3d: timing {name}")
+        for _ in range(args.warmup):
+            plan.solve(F, out=out)
+        torch.cuda.synchronize()
+        lib = H.lib()
+        lib.hh_context_reset_timing(ctx.h)
+        lib.hh_context_enable_timing(ctx.h, 1)
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for a, b in ev:
+            a.record()
+            plan.solve(F, out=out)
+            b.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lib.hh_context_enable_timing(ctx.h, 0)
+        lat = [a.elapsed_time(b) for a, b in ev]
+        res[name] = {"dof_per_s": N ** 3 * args.steps / wall, "ms_per_solve": 1e3 * wall / args.steps,
+                     "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                     "kernels": _timing(H, ctx)}
+    sw, rel = pmx.last_refine()
+    res["mixed"].update(sweeps=sw, rel_residual=rel)
+    # exact solution check on the GPU (norms kernel), oracle check on the host
+    for name, out in (("fp64", s64), ("mixed", smx)):
+        m, d = ctx.norms(out, u_exact)
+        res[name]["disc_err_vs_exact"] = d / max(m, 1e-300)
+    cpu = None
+    if not (args.no_check and args.no_cpu):
+        from oracle import oracle as O
+        log("3d: oracle solve on the host")
+        Fh = F.cpu().numpy().ravel()
+        threads = max(1, min(len(os.sched_getaffinity(0)), 16))
+        t0 = time.perf_counter()
+        LUU = O.lu3d(N, N, N, coef)
+        ref = O.solver3d(N, N, N, Fh, LUU, threads)
+        t_cpu = time.perf_counter() - t0
+        for name, out in (("fp64", s64), ("mixed", smx)):
+            g = out.cpu().numpy().ravel()
+            res[name]["max_rel_err_vs_oracle"] = float(np.abs(g - ref).max() / np.abs(ref).max())
+        if not args.no_cpu:
+            cpu = {"value": N ** 3 / t_cpu, "unit": "complex DOF/s", "cores": threads, "kind": "port",
+                   "sample": f"oracle LU + solver() of the 3D reference CPU path (OpenMP over layers/modes) on one "
+                             f"right-hand side of the same {N}^3 k=30 problem, {t_cpu:.1f} s"}
+    mixed_ok = res["mixed"].get("max_rel_err_vs_oracle", 0.0) <= 1e-10
+    value = res["mixed"]["dof_per_s"] if mixed_ok else res["fp64"]["dof_per_s"]
+    return _dof_line(" (3D)", value, "complex DOF/s", args.steps, args.warmup, res["mixed"]["ms_per_solve"],
+                     {"workload": f"3D compact Helmholtz N={N}^3, k=30, order 6, u=(1+i) sin(a pi x) sin(b pi y) "
+                                  f"sin(c pi z), (a,b,c)={tuple(abc)} from seed {args.seed}, GPU RHS assembly",
+                      "grid": [N, N, N], "orders": [order]},
+                     dtype="f32+f64" if mixed_ok else "f64", value_path="mixed" if mixed_ok else "fp64",
+                     mixed_over_fp64=res["mixed"]["dof_per_s"] / res["fp64"]["dof_per_s"], results=res,
+                     cpu_baseline=cpu)
+
+
+def run_uneven(args, torch, H):
+    """512 uneven 2D problems (N log-uniform in 63..2047, k in [1,80], order 4/6), FP32 bulk + FP64
+    refinement with fixed sweeps, grouped by (N, order) into multi-operator plans and replayed as one
+    HIP graph."""
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(args.seed)
+    Ns = np.array([63, 127, 255, 511, 1023, 2047])
+    P = args.problems
+    Nsel = rng.choice(Ns, size=P)
+    ksel = rng.uniform(1.0, 80.0, size=P)
+    osel = rng.choice([4, 6], size=P)
+    ctx = H.Context(0)
+    groups = {}
+    for i in range(P):
+        groups.setdefault((int(Nsel[i]), int(osel[i])), []).append(i)
+    solves64, solvesmx, info = [], [], []
+    total_dof = 0
+    for (N, order), idx in sorted(groups.items()):
+        h = 1.0 / (N + 1)
+        ks = ksel[idx]
+        k2es = [np.full(N + 2, k * k, np.complex128) for k in ks]
+        ops = [H.coefficients2d(order, N, N, h, h, k2e) for k2e in k2es]
+        p64 = H.Plan2D(ctx, N, N, ops)
+        pmx = H.Plan2D(ctx, N, N, ops, precision=H.PREC_MIXED)
+        pmx.set_refine(0.0, args.sweeps)
+        B = len(idx)
+        m = manufactured(torch, N, B, 0.0, args.seed + N + order, dev)
+        # per-problem k: f = (k^2 - pi^2 (a^2 + b^2)) u, rebuilt from u
+        fac = torch.tensor(ks ** 2 - np.pi ** 2 * (m["a"] ** 2 + m["b"] ** 2), dtype=torch.complex128, device=dev)
+        f_e = m["u_e"] * fac[:, None, None]
+        x = torch.arange(N + 2, dtype=torch.float64, device=dev) * h
+        A = torch.tensor(m["a"], dtype=torch.float64, device=dev)[:, None]
+        Bb = torch.tensor(m["b"], dtype=torch.float64, device=dev)[:, None]
+        amp = torch.tensor(np.exp(1j * m["phi"]), dtype=torch.complex128, device=dev)
+        sx = torch.sin(A * math.pi * x[None, :])
+        cy = torch.cos(Bb * math.pi * x[None, :])
+        fy_b = torch.cat([(fac * amp)[:, None] * sx[:, 1:-1] * (Bb * math.pi) * cy[:, :1],
+                          (fac * amp)[:, None] * sx[:, 1:-1] * (Bb * math.pi) * cy[:, -1:]], dim=1).contiguous()
+
+        def ring(g, s):
+            return torch.cat([-s * g[:, 0, :], -s * g[:, -1, :], -s * g[:, :, 0], -s * g[:, :, -1]],
+                             dim=1).contiguous()
+
+        fxx_b = ring(f_e, ((A * math.pi) ** 2).to(torch.complex128))
+        fyy_b = ring(f_e, ((Bb * math.pi) ** 2).to(torch.complex128))
+        F = p64.rhs(order, h, h, f_e.contiguous(), m["u_e"], np.concatenate(k2es), fy_b=fy_b, fxx_b=fxx_b,
+                    fyy_b=fyy_b, k2yb=np.zeros(2 * B))
+        u_ex = m["u_e"][:, 1:-1, 1:-1].contiguous()
+        del m, f_e
+        solves64.append((p64, F, torch.empty_like(F)))
+        solvesmx.append((pmx, F, torch.empty_like(F)))
+        info.append({"N": N, "order": order, "problems": B, "k": ks, "ops": ops, "u": u_ex})
+        total_dof += B * N * N
+        log(f"uneven: group N={N} order={order}: {B} problems")
+    torch.cuda.synchronize()
+    out = {}
+    for name, solves in (("fp64", solves64), ("mixed", solvesmx)):
+        log(f"uneven: recording and timing the {name} graph")
+        g = H.Graph(ctx, solves)
+        for _ in range(args.warmup):
+            g.launch()
+        torch.cuda.synchronize()
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+        t0 = time.perf_counter()
+        for a, b in ev:
+            a.record()
+            g.launch()
+            b.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lat = [a.elapsed_time(b) for a, b in ev]
+        out[name] = {"dof_per_s": total_dof * args.steps / wall, "ms_per_launch": 1e3 * wall / args.steps,
+                     "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                     "graph_nodes": g.num_nodes}
+        # discretisation error of every problem against its exact solution
+        worst = 0.0
+        for (plan, F, sol), gi in zip(solves, info):
+            mx, d = ctx.norms(sol, gi["u"])
+            worst = max(worst, d / mx)
+        out[name]["max_disc_err_vs_exact"] = worst
+        g.close()
+    check = {}
+    if not args.no_check:
+        from oracle import oracle as O
+        log("uneven: oracle checks")
+        for (pmx, F, sol), gi in zip(solvesmx, info):
+            if gi["N"] > 1023:
+                continue
+            N = gi["N"]
+            LUU = O.lu2d(N, N, gi["ops"][0], 16)
+            ref = O.solver2d(N, N, F[0].cpu().numpy().ravel(), LUU, 16)
+            g = sol[0].cpu().numpy().ravel()
+            e = float(np.abs(g - ref).max() / np.abs(ref).max())
+            check[f"N{N}_order{gi['order']}"] = e
+        out["mixed"]["max_rel_err_vs_oracle"] = max(check.values()) if check else None
+    ok = out["mixed"].get("max_rel_err_vs_oracle") is not None and out["mixed"]["max_rel_err_vs_oracle"] <= 1e-10
+    best = "mixed" if ok and out["mixed"]["dof_per_s"] >= out["fp64"]["dof_per_s"] else "fp64"
+    sizes = {str(k): len(v) for k, v in sorted(groups.items())}
+    return _dof_line(" (uneven batch)", out[best]["dof_per_s"], "complex DOF/s", args.steps, args.warmup,
+                     out[best]["ms_per_launch"],
+                     {"workload": f"{P} uneven 2D problems, N log-uniform in {{63..2047}}, k~U[1,80], order~U{{4,6}}, "
+                                  f"manufactured sines (seed {args.seed}); one HIP graph launch per step",
+                      "problems": P, "total_dof": total_dof, "groups": sizes},
+                     dtype="f32+f64" if best == "mixed" else "f64", value_path=best,
+                     mixed_over_fp64=out["mixed"]["dof_per_s"] / out["fp64"]["dof_per_s"], results=out,
+                     oracle_check=check)
+
+
+def run_serve(args, torch, H):
+    """Poisson request stream of mixed 2D/3D solves through the continuous-batching server."""
+    import threading
+    rng = np.random.default_rng(args.seed)
+    shapes2 = [127, 255, 511, 1023, 2047]
+    shapes3 = [63, 127, 255]
+    pools = {}
+    for d, Ns in ((2, shapes2), (3, shapes3)):
+        for N in Ns:
+            n = N ** d
+            pools[(d, N)] = [(rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex128)
+                             for _ in range(2)]
+    reqs = []
+    for _ in range(args.requests):
+        if rng.uniform() < 0.7:
+            d, N = 2, int(rng.choice(shapes2))
+        else:
+            d, N = 3, int(rng.choice(shapes3))
+        reqs.append((d, N, int(rng.choice([4, 6])), float(rng.uniform(1.0, 60.0)), int(rng.integers(0, 2))))
+    gaps = rng.exponential(1.0 / args.qps, size=len(reqs))
+    srv = H.Server(0, hbm_budget=96 << 30, lanes=args.lanes, max_batch=8)
+    # warm-up: every grid once per lane (plans, tables, code objects)
+    warm = [srv.submit(d, N, 4, 10.0, pools[(d, N)][0]) for (d, N) in pools for _ in range(args.lanes)]
+    for t in warm:
+        srv.wait(t)
+    log(f"serve: warm-up of {len(warm)} requests done; streaming {len(reqs)} at {args.qps:g}/s")
+    st0 = srv.stats()
+    results = [None] * len(reqs)
+    tickets = [None] * len(reqs)
+    ready = [threading.Event() for _ in reqs]
+
+    def waiter():
+        for i in range(len(reqs)):
+            ready[i].wait()
+            sol, info = srv.wait(tickets[i])
+            results[i] = (sol, info, time.perf_counter())
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t_start = time.perf_counter()
+    t_next = t_start
+    for i, (d, N, order, k, which) in enumerate(reqs):
+        t_next += gaps[i]
+        delay = t_next - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        tickets[i] = srv.submit(d, N, order, k, pools[(d, N)][which])
+        ready[i].set()
+    th.join()
+    t_end = max(r[2] for r in results)
+    wall = t_end - t_start
+    st1 = srv.stats()
+    lat = np.array([r[1]["latency_s"] for r in results]) * 1e3
+    q = np.array([r[1]["queue_s"] for r in results]) * 1e3
+    dof = sum(N ** d for d, N, _, _, _ in reqs)
+    check = {}
+    if not args.no_check:
+        from oracle import oracle as O
+        seen = set()
+        for i, (d, N, order, k, which) in enumerate(reqs):
+            if d != 2 or N > 1023 or (N, order) in seen or len(seen) >= 4:
+                continue
+            seen.add((N, order))
+            h = 1.0 / (N + 1)
+            coef = H.coefficients2d(order, N, N, h, h, np.full(N + 2, k * k, np.complex128))
+            ref = O.solver2d(N, N, pools[(d, N)][which], O.lu2d(N, N, coef, 16), 16)
+            g = results[i][0].ravel()
+            check[f"2D N{N} order{order} k{k:.1f}"] = float(np.abs(g - ref).max() / np.abs(ref).max())
+    srv.close()
+    return _dof_line(" (serving)", dof / wall, "complex DOF/s", len(reqs), len(warm), 1e3 * wall / len(reqs),
+                     {"workload": f"Poisson stream, {args.qps:g} req/s, {len(reqs)} requests: 70% 2D N in "
+                                  f"{{127..2047}}, 30% 3D N in {{63,127,255}}, k~U[1,60], order~U{{4,6}}, FP64, "
+                                  f"host right-hand sides (seed {args.seed}); {args.lanes} lanes, max batch 8",
+                      "qps": args.qps, "lanes": args.lanes},
+                     p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                     p50_queue_ms=float(np.percentile(q, 50)), p99_queue_ms=float(np.percentile(q, 99)),
+                     latency_unit="arrival to solution in host memory (includes PCIe copies)",
+                     requests_per_s=len(reqs) / wall, batches=st1["batches"] - st0["batches"],
+                     oracle_check=check,
+                     max_rel_err_vs_oracle=max(check.values()) if check else None)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -111,6 +423,10 @@ def main():
     from hackathon_amd import helmholtz as H
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.workload != "c2":
+        out = {"3d": run_3d, "uneven": run_uneven, "serve": run_serve}[args.workload](args, torch, H)
+        print(json.dumps(out), flush=True)
+        return
     dev = torch.device("cuda:0")
     N, B, kk = args.N, args.batch, args.k
     orders = [int(o) for o in args.orders.split(",")]

@@ -49,7 +49,7 @@ struct hh_context {
     std::map<int, void *> tw64, tw32;   // twiddles by FFT length
     std::map<int, void *> S64, S32;     // dense DST matrices by N
     std::map<int, void *> th64, th32;   // half-length transform tables by N+1
-    int dst_algo = 0;                   // 0: (N+1)-point transform, 1: odd extension
+    int dst_algo = 0;                   // bit 0: odd extension (else (N+1)-point), bit 1: 64-thread blocks
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     double *d_norm = nullptr;
@@ -236,7 +236,8 @@ static int dst_pass(hh_context *c, const hh::Lines &ln, int ti, int tc, int to, 
     void *tw = nullptr, *S = nullptr;
     int rc;
     const int algo = c->dst_algo;
-    if (hh::fft_supported(ln.N)) rc = algo == 0 ? get_half_table(c, ln.N, tc, &tw) : get_twiddles(c, ln.N, tc, &tw);
+    if (hh::fft_supported(ln.N))
+        rc = (algo & 1) == 0 ? get_half_table(c, ln.N, tc, &tw) : get_twiddles(c, ln.N, tc, &tw);
     else rc = get_dst_matrix(c, ln.N, tc, &S);
     if (rc) return rc;
     const double elems = (double)ln.N * ln.ncols * ln.nplanes;
@@ -362,7 +363,7 @@ static int collect_timing(hh_context *c) {
 
 extern "C" int hh_context_set_dst_algorithm(hh_context *c, int algo) {
     if (!c) return fail(HH_ERR_ARG, "null context");
-    if (algo != 0 && algo != 1) return fail(HH_ERR_ARG, "DST algorithm must be 0 or 1 (got %d)", algo);
+    if (algo < 0 || algo > 3) return fail(HH_ERR_ARG, "DST algorithm must be 0..3 (got %d)", algo);
     std::lock_guard<std::mutex> lk(c->mu);
     c->dst_algo = algo;
     return HH_OK;
@@ -882,7 +883,8 @@ static int prepare_tables(hh_plan *p) {
         if (N <= 0) continue;
         void *t = nullptr;
         int rc;
-        if (hh::fft_supported(N)) rc = c->dst_algo == 0 ? get_half_table(c, N, tc, &t) : get_twiddles(c, N, tc, &t);
+        if (hh::fft_supported(N))
+            rc = (c->dst_algo & 1) == 0 ? get_half_table(c, N, tc, &t) : get_twiddles(c, N, tc, &t);
         else rc = get_dst_matrix(c, N, tc, &t);
         if (rc) return rc;
     }

@@ -29,18 +29,21 @@ template <typename T> struct tsel;
 template <> struct tsel<double> { static constexpr int id = 0; };
 template <> struct tsel<float> { static constexpr int id = 1; };
 
-template <int L> struct dst_cfg {
+// TPB: target threads per block (256, or 64 for one-wave blocks whose
+// barriers are nearly free and whose small LDS footprint lets more blocks
+// share a CU)
+template <int L, int TPB = 256> struct dst_cfg {
     static constexpr int P = fft_shape<L>::P;
-    static constexpr int LPB = P >= 256 ? 1 : 256 / P;   // lines per block
+    static constexpr int LPB = P >= TPB ? 1 : TPB / P;   // lines per block
     static constexpr int THREADS = P * LPB;
     static constexpr int LP = fft_shape<L>::LP;
 };
 
-template <typename Ti, typename T, typename To, int L>
-__global__ __launch_bounds__(dst_cfg<L>::THREADS) void k_dst_fft_rows(
+template <typename Ti, typename T, typename To, int L, int TPB>
+__global__ __launch_bounds__((dst_cfg<L, TPB>::THREADS)) void k_dst_fft_rows(
     const C2<Ti> *in, C2<To> *out, long nrows, T scale,
     const C2<T> *__restrict__ tw) {
-    constexpr int P = dst_cfg<L>::P, LPB = dst_cfg<L>::LPB, LP = dst_cfg<L>::LP;
+    constexpr int P = dst_cfg<L, TPB>::P, LPB = dst_cfg<L, TPB>::LPB, LP = dst_cfg<L, TPB>::LP;
     constexpr int M = L / 2, N = M - 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *smem = reinterpret_cast<T *>(smem_raw);
@@ -107,12 +110,12 @@ HH_DEV void store_col_tile(const T *smem, C2<To> *out, long base, long c0, long 
     }
 }
 
-template <typename Ti, typename T, typename To, int L>
-__global__ __launch_bounds__(dst_cfg<L>::THREADS) void k_dst_fft_cols(
+template <typename Ti, typename T, typename To, int L, int TPB>
+__global__ __launch_bounds__((dst_cfg<L, TPB>::THREADS)) void k_dst_fft_cols(
     const C2<Ti> *in, C2<To> *out, long ncols, long ntiles,
     long plane_stride, long elem_stride, T scale, const C2<T> *__restrict__ tw) {
-    constexpr int P = dst_cfg<L>::P, LPB = dst_cfg<L>::LPB, LP = dst_cfg<L>::LP;
-    constexpr int M = L / 2, N = M - 1, NT = dst_cfg<L>::THREADS;
+    constexpr int P = dst_cfg<L, TPB>::P, LPB = dst_cfg<L, TPB>::LPB, LP = dst_cfg<L, TPB>::LP;
+    constexpr int M = L / 2, N = M - 1, NT = dst_cfg<L, TPB>::THREADS;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *smem = reinterpret_cast<T *>(smem_raw);
     const long tile = blockIdx.x % ntiles, plane = blockIdx.x / ntiles;
@@ -152,9 +155,9 @@ __global__ __launch_bounds__(dst_cfg<L>::THREADS) void k_dst_fft_cols(
 // the LDS of the odd-extension transform; the odd outputs need a prefix sum
 // over M/2 terms, done with wave shuffles inside a line and through LDS
 // across the waves of a line.
-template <int M> struct half_cfg {
+template <int M, int TPB = 256> struct half_cfg {
     static constexpr int P = fft_shape<M>::P;
-    static constexpr int LPB = P >= 256 ? 1 : 256 / P;
+    static constexpr int LPB = P >= TPB ? 1 : TPB / P;
     static constexpr int THREADS = P * LPB;
     static constexpr int LP = fft_shape<M>::LP;
     static constexpr int K = (M / 2) / P;   // pairs (F_k, F_{M-k}) per thread, M >= 2P
@@ -218,10 +221,12 @@ HH_DEV void half_post(T *sre, T *sim, T *tot_re, T *tot_im, int t, T scale) {
 }
 
 // rows (contiguous lines); aux = [twiddles of length M | sin(pi j/M), j < M]
-template <typename Ti, typename T, typename To, int M>
-__global__ __launch_bounds__(half_cfg<M>::THREADS) void k_dst_half_rows(const C2<Ti> *in, C2<To> *out, long nrows,
-                                                                       T scale, const C2<T> *__restrict__ aux) {
-    constexpr int P = half_cfg<M>::P, LPB = half_cfg<M>::LPB, LP = half_cfg<M>::LP, N = M - 1, NT = half_cfg<M>::THREADS;
+template <typename Ti, typename T, typename To, int M, int TPB>
+__global__ __launch_bounds__((half_cfg<M, TPB>::THREADS)) void k_dst_half_rows(const C2<Ti> *in, C2<To> *out,
+                                                                            long nrows, T scale,
+                                                                            const C2<T> *__restrict__ aux) {
+    using cfg = half_cfg<M, TPB>;
+    constexpr int P = cfg::P, LPB = cfg::LPB, LP = cfg::LP, N = M - 1, NT = cfg::THREADS;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *smem = reinterpret_cast<T *>(smem_raw);
     T *tot = smem + 2 * LPB * LP;   // per line: 2 * (P/64 + 1) reals
@@ -263,11 +268,13 @@ __global__ __launch_bounds__(half_cfg<M>::THREADS) void k_dst_half_rows(const C2
     }
 }
 
-template <typename Ti, typename T, typename To, int M>
-__global__ __launch_bounds__(half_cfg<M>::THREADS) void k_dst_half_cols(const C2<Ti> *in, C2<To> *out, long ncols,
-                                                                       long ntiles, long plane_stride, long elem_stride,
-                                                                       T scale, const C2<T> *__restrict__ aux) {
-    constexpr int P = half_cfg<M>::P, LPB = half_cfg<M>::LPB, LP = half_cfg<M>::LP, N = M - 1, NT = half_cfg<M>::THREADS;
+template <typename Ti, typename T, typename To, int M, int TPB>
+__global__ __launch_bounds__((half_cfg<M, TPB>::THREADS)) void k_dst_half_cols(const C2<Ti> *in, C2<To> *out,
+                                                                            long ncols, long ntiles, long plane_stride,
+                                                                            long elem_stride, T scale,
+                                                                            const C2<T> *__restrict__ aux) {
+    using cfg = half_cfg<M, TPB>;
+    constexpr int P = cfg::P, LPB = cfg::LPB, LP = cfg::LP, N = M - 1, NT = cfg::THREADS;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *smem = reinterpret_cast<T *>(smem_raw);
     T *tot = smem + 2 * LPB * LP;
@@ -362,9 +369,9 @@ bool fft_supported(int N) {
     return N >= 1 && L <= HH_FFT_LMAX && (L & (L - 1)) == 0 && L >= 4;
 }
 
-template <typename Ti, typename T, typename To, int L>
+template <typename Ti, typename T, typename To, int L, int TPB>
 static hipError_t launch_fft_L(const Lines &ln, const void *in, void *out, const void *tw, hipStream_t st) {
-    using cfg = dst_cfg<L>;
+    using cfg = dst_cfg<L, TPB>;
     size_t shm = sizeof(T) * 2 * cfg::LPB * cfg::LP;
     T scale = (T)sqrt(0.5 / (double)(L / 2));
     const C2<Ti> *pi = (const C2<Ti> *)in;
@@ -373,7 +380,7 @@ static hipError_t launch_fft_L(const Lines &ln, const void *in, void *out, const
     if (ln.elem_stride == 1 && ln.col_stride == ln.N && ln.nplanes * ln.ncols > 0 &&
         (ln.nplanes == 1 || ln.plane_stride == ln.ncols * ln.N)) {
         long nrows = ln.ncols * ln.nplanes;
-        auto kfn = k_dst_fft_rows<Ti, T, To, L>;
+        auto kfn = k_dst_fft_rows<Ti, T, To, L, TPB>;
         if (shm > 65536) {
             hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
             if (e != hipSuccess) return e;
@@ -383,7 +390,7 @@ static hipError_t launch_fft_L(const Lines &ln, const void *in, void *out, const
         return hipGetLastError();
     }
     if (ln.col_stride != 1) return hipErrorInvalidValue;
-    auto kfn = k_dst_fft_cols<Ti, T, To, L>;
+    auto kfn = k_dst_fft_cols<Ti, T, To, L, TPB>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
@@ -395,9 +402,9 @@ static hipError_t launch_fft_L(const Lines &ln, const void *in, void *out, const
     return hipGetLastError();
 }
 
-template <typename Ti, typename T, typename To, int M>
+template <typename Ti, typename T, typename To, int M, int TPB>
 static hipError_t launch_half_M(const Lines &ln, const void *in, void *out, const void *aux, hipStream_t st) {
-    using cfg = half_cfg<M>;
+    using cfg = half_cfg<M, TPB>;
     constexpr int NW = cfg::P > 64 ? cfg::P / 64 : 1;
     size_t shm = sizeof(T) * (2 * cfg::LPB * cfg::LP + 2 * cfg::LPB * NW);
     T scale = (T)sqrt(2.0 / (double)M);
@@ -406,7 +413,7 @@ static hipError_t launch_half_M(const Lines &ln, const void *in, void *out, cons
     const C2<T> *pa = (const C2<T> *)aux;
     if (ln.elem_stride == 1 && ln.col_stride == ln.N && (ln.nplanes == 1 || ln.plane_stride == ln.ncols * ln.N)) {
         long nrows = ln.ncols * ln.nplanes;
-        auto kfn = k_dst_half_rows<Ti, T, To, M>;
+        auto kfn = k_dst_half_rows<Ti, T, To, M, TPB>;
         if (shm > 65536) {
             hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
             if (e != hipSuccess) return e;
@@ -416,7 +423,7 @@ static hipError_t launch_half_M(const Lines &ln, const void *in, void *out, cons
         return hipGetLastError();
     }
     if (ln.col_stride != 1) return hipErrorInvalidValue;
-    auto kfn = k_dst_half_cols<Ti, T, To, M>;
+    auto kfn = k_dst_half_cols<Ti, T, To, M, TPB>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
@@ -428,42 +435,50 @@ static hipError_t launch_half_M(const Lines &ln, const void *in, void *out, cons
     return hipGetLastError();
 }
 
-template <typename Ti, typename T, typename To>
-static hipError_t launch_dst_t(const Lines &ln, int algo, const void *in, void *out, const void *tw, const void *S,
-                               hipStream_t st) {
-    if (fft_supported(ln.N) && algo == 0) {
+template <typename Ti, typename T, typename To, int TPB>
+static hipError_t launch_fft_t(const Lines &ln, int odd, const void *in, void *out, const void *tw, hipStream_t st) {
+    if (!odd) {
         switch (ln.N + 1) {
-        case 2: return launch_half_M<Ti, T, To, 2>(ln, in, out, tw, st);
-        case 4: return launch_half_M<Ti, T, To, 4>(ln, in, out, tw, st);
-        case 8: return launch_half_M<Ti, T, To, 8>(ln, in, out, tw, st);
-        case 16: return launch_half_M<Ti, T, To, 16>(ln, in, out, tw, st);
-        case 32: return launch_half_M<Ti, T, To, 32>(ln, in, out, tw, st);
-        case 64: return launch_half_M<Ti, T, To, 64>(ln, in, out, tw, st);
-        case 128: return launch_half_M<Ti, T, To, 128>(ln, in, out, tw, st);
-        case 256: return launch_half_M<Ti, T, To, 256>(ln, in, out, tw, st);
-        case 512: return launch_half_M<Ti, T, To, 512>(ln, in, out, tw, st);
-        case 1024: return launch_half_M<Ti, T, To, 1024>(ln, in, out, tw, st);
-        case 2048: return launch_half_M<Ti, T, To, 2048>(ln, in, out, tw, st);
-        case 4096: return launch_half_M<Ti, T, To, 4096>(ln, in, out, tw, st);
+        case 2: return launch_half_M<Ti, T, To, 2, TPB>(ln, in, out, tw, st);
+        case 4: return launch_half_M<Ti, T, To, 4, TPB>(ln, in, out, tw, st);
+        case 8: return launch_half_M<Ti, T, To, 8, TPB>(ln, in, out, tw, st);
+        case 16: return launch_half_M<Ti, T, To, 16, TPB>(ln, in, out, tw, st);
+        case 32: return launch_half_M<Ti, T, To, 32, TPB>(ln, in, out, tw, st);
+        case 64: return launch_half_M<Ti, T, To, 64, TPB>(ln, in, out, tw, st);
+        case 128: return launch_half_M<Ti, T, To, 128, TPB>(ln, in, out, tw, st);
+        case 256: return launch_half_M<Ti, T, To, 256, TPB>(ln, in, out, tw, st);
+        case 512: return launch_half_M<Ti, T, To, 512, TPB>(ln, in, out, tw, st);
+        case 1024: return launch_half_M<Ti, T, To, 1024, TPB>(ln, in, out, tw, st);
+        case 2048: return launch_half_M<Ti, T, To, 2048, TPB>(ln, in, out, tw, st);
+        case 4096: return launch_half_M<Ti, T, To, 4096, TPB>(ln, in, out, tw, st);
         default: return hipErrorInvalidValue;
         }
     }
+    switch (2 * (ln.N + 1)) {
+    case 4: return launch_fft_L<Ti, T, To, 4, TPB>(ln, in, out, tw, st);
+    case 8: return launch_fft_L<Ti, T, To, 8, TPB>(ln, in, out, tw, st);
+    case 16: return launch_fft_L<Ti, T, To, 16, TPB>(ln, in, out, tw, st);
+    case 32: return launch_fft_L<Ti, T, To, 32, TPB>(ln, in, out, tw, st);
+    case 64: return launch_fft_L<Ti, T, To, 64, TPB>(ln, in, out, tw, st);
+    case 128: return launch_fft_L<Ti, T, To, 128, TPB>(ln, in, out, tw, st);
+    case 256: return launch_fft_L<Ti, T, To, 256, TPB>(ln, in, out, tw, st);
+    case 512: return launch_fft_L<Ti, T, To, 512, TPB>(ln, in, out, tw, st);
+    case 1024: return launch_fft_L<Ti, T, To, 1024, TPB>(ln, in, out, tw, st);
+    case 2048: return launch_fft_L<Ti, T, To, 2048, TPB>(ln, in, out, tw, st);
+    case 4096: return launch_fft_L<Ti, T, To, 4096, TPB>(ln, in, out, tw, st);
+    case 8192: return launch_fft_L<Ti, T, To, 8192, TPB>(ln, in, out, tw, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// algo bit 0: odd-extension transform (else half length); bit 1: 64-thread
+// blocks (else 256)
+template <typename Ti, typename T, typename To>
+static hipError_t launch_dst_t(const Lines &ln, int algo, const void *in, void *out, const void *tw, const void *S,
+                               hipStream_t st) {
     if (fft_supported(ln.N)) {
-        switch (2 * (ln.N + 1)) {
-        case 4: return launch_fft_L<Ti, T, To, 4>(ln, in, out, tw, st);
-        case 8: return launch_fft_L<Ti, T, To, 8>(ln, in, out, tw, st);
-        case 16: return launch_fft_L<Ti, T, To, 16>(ln, in, out, tw, st);
-        case 32: return launch_fft_L<Ti, T, To, 32>(ln, in, out, tw, st);
-        case 64: return launch_fft_L<Ti, T, To, 64>(ln, in, out, tw, st);
-        case 128: return launch_fft_L<Ti, T, To, 128>(ln, in, out, tw, st);
-        case 256: return launch_fft_L<Ti, T, To, 256>(ln, in, out, tw, st);
-        case 512: return launch_fft_L<Ti, T, To, 512>(ln, in, out, tw, st);
-        case 1024: return launch_fft_L<Ti, T, To, 1024>(ln, in, out, tw, st);
-        case 2048: return launch_fft_L<Ti, T, To, 2048>(ln, in, out, tw, st);
-        case 4096: return launch_fft_L<Ti, T, To, 4096>(ln, in, out, tw, st);
-        case 8192: return launch_fft_L<Ti, T, To, 8192>(ln, in, out, tw, st);
-        default: return hipErrorInvalidValue;
-        }
+        if (algo & 2) return launch_fft_t<Ti, T, To, 64>(ln, algo & 1, in, out, tw, st);
+        return launch_fft_t<Ti, T, To, 256>(ln, algo & 1, in, out, tw, st);
     }
     if (in == out || S == nullptr) return hipErrorInvalidValue;
     long nlines = ln.ncols * ln.nplanes;

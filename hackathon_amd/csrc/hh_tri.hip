@@ -155,6 +155,7 @@ HH_DEV long xcd_tile(long b, long G) {
 template <typename T, int MMAX, int NT, class Prob>
 __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    // buffer b of the reduced rows: A at rA + b*RW, C at rC + b*RW, D at rD + b*RW
     C2<T> *rA = reinterpret_cast<C2<T> *>(smem_raw);
     C2<T> *rC = rA + 2 * P * W;
     C2<T> *rD = rC + 2 * P * W;
@@ -232,40 +233,41 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
         A0 = C0 = A1 = C1 = mk<T>(0, 0);
         D0 = D1 = mk<T>(0, 0);
     }
-    // 3. PCR on the reduced system, rows r0 = 2c, r1 = 2c+1, layout [row][W]
+    // 3. PCR on the reduced system, rows r0 = 2c, r1 = 2c+1, layout [row][W].
+    //    Two LDS buffers: step i reads buffer i%2 and writes the other, so
+    //    one barrier per step suffices.
     const int R = 2 * P;
     const int r0 = 2 * c, r1 = 2 * c + 1;
+    const int RW = 3 * R * W;   // complex entries per buffer (A, C, D)
     if (c < P) {
         rA[r0 * W + w] = A0; rC[r0 * W + w] = C0; rD[r0 * W + w] = D0;
         rA[r1 * W + w] = A1; rC[r1 * W + w] = C1; rD[r1 * W + w] = D1;
     }
+    int cur = 0;
     for (int dd = 1; dd < R; dd <<= 1) {
         __syncthreads();
-        C2<T> z = mk<T>(0, 0);
-        C2<T> aL0 = z, cL0 = z, dL0 = z, aU0 = z, cU0 = z, dU0 = z;
-        C2<T> aL1 = z, cL1 = z, dL1 = z, aU1 = z, cU1 = z, dU1 = z;
+        const C2<T> *sA = rA + cur * RW, *sC = rC + cur * RW, *sD = rD + cur * RW;
+        C2<T> *tA = rA + (cur ^ 1) * RW, *tC = rC + (cur ^ 1) * RW, *tD = rD + (cur ^ 1) * RW;
         if (c < P) {
-            int q;
-            q = r0 - dd; if (q >= 0) { aL0 = rA[q * W + w]; cL0 = rC[q * W + w]; dL0 = rD[q * W + w]; }
-            q = r0 + dd; if (q < R) { aU0 = rA[q * W + w]; cU0 = rC[q * W + w]; dU0 = rD[q * W + w]; }
-            q = r1 - dd; if (q >= 0) { aL1 = rA[q * W + w]; cL1 = rC[q * W + w]; dL1 = rD[q * W + w]; }
-            q = r1 + dd; if (q < R) { aU1 = rA[q * W + w]; cU1 = rC[q * W + w]; dU1 = rD[q * W + w]; }
+            // row r: A x_{r-d} + x_r + C x_{r+d} = D, eliminated against rows r-d and r+d
+            auto step = [&](int r, C2<T> &A, C2<T> &C, C2<T> &D) {
+                const C2<T> z = mk<T>(0, 0);
+                C2<T> aL = z, cL = z, dL = z, aU = z, cU = z, dU = z;
+                int q = r - dd;
+                if (q >= 0) { aL = sA[q * W + w]; cL = sC[q * W + w]; dL = sD[q * W + w]; }
+                q = r + dd;
+                if (q < R) { aU = sA[q * W + w]; cU = sC[q * W + w]; dU = sD[q * W + w]; }
+                const C2<T> b = csub<T>(mk<T>(1, 0), cadd<T>(cmul<T>(A, cL), cmul<T>(C, aU)));
+                const C2<T> na = cneg<T>(cmul<T>(A, aL)), nc = cneg<T>(cmul<T>(C, cU));
+                const C2<T> nd = csub<T>(D, cadd<T>(cmul<T>(A, dL), cmul<T>(C, dU)));
+                const C2<T> ib = crcp<T>(b);
+                A = cmul<T>(na, ib); C = cmul<T>(nc, ib); D = cmul<T>(nd, ib);
+                tA[r * W + w] = A; tC[r * W + w] = C; tD[r * W + w] = D;
+            };
+            step(r0, A0, C0, D0);
+            step(r1, A1, C1, D1);
         }
-        __syncthreads();
-        if (c < P) {
-            // row: A x_{r-d} + x_r + C x_{r+d} = D
-            C2<T> b0 = csub<T>(mk<T>(1, 0), cadd<T>(cmul<T>(A0, cL0), cmul<T>(C0, aU0)));
-            C2<T> na0 = cneg<T>(cmul<T>(A0, aL0)), nc0 = cneg<T>(cmul<T>(C0, cU0));
-            C2<T> nd0 = csub<T>(D0, cadd<T>(cmul<T>(A0, dL0), cmul<T>(C0, dU0)));
-            C2<T> b1 = csub<T>(mk<T>(1, 0), cadd<T>(cmul<T>(A1, cL1), cmul<T>(C1, aU1)));
-            C2<T> na1 = cneg<T>(cmul<T>(A1, aL1)), nc1 = cneg<T>(cmul<T>(C1, cU1));
-            C2<T> nd1 = csub<T>(D1, cadd<T>(cmul<T>(A1, dL1), cmul<T>(C1, dU1)));
-            C2<T> ib0 = crcp<T>(b0), ib1 = crcp<T>(b1);
-            A0 = cmul<T>(na0, ib0); C0 = cmul<T>(nc0, ib0); D0 = cmul<T>(nd0, ib0);
-            A1 = cmul<T>(na1, ib1); C1 = cmul<T>(nc1, ib1); D1 = cmul<T>(nd1, ib1);
-            rA[r0 * W + w] = A0; rC[r0 * W + w] = C0; rD[r0 * W + w] = D0;
-            rA[r1 * W + w] = A1; rC[r1 * W + w] = C1; rD[r1 * W + w] = D1;
-        }
+        cur ^= 1;
     }
     if (!valid) return;
     const C2<T> xs = D0, xe = D1;
@@ -287,7 +289,7 @@ static hipError_t launch_tri_prob(const Prob &pb, int n, long nmodes, int batch,
     // every chunk needs at least two rows, at most MM rows
     if (P < 1 || P > n / 2 || (n + P - 1) / P > MM || W * P > 1024) return hipErrorInvalidValue;
     int threads = ((W * P + 63) / 64) * 64;
-    size_t shm = sizeof(C2<T>) * 3 * 2 * (size_t)P * W;
+    size_t shm = sizeof(C2<T>) * 2 * 3 * 2 * (size_t)P * W;   // two buffers of A, C, D
     // 512-thread bound: 256 VGPRs per lane, no spills of the chunk registers
     // up to 8 FP64 rows per thread; 16 FP64 rows need the 256-thread bound
     // (one wave per SIMD, the whole register file) to stay out of scratch

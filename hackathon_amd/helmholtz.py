@@ -98,7 +98,7 @@ class Context:
         _check(lib().hh_context_synchronize(self.h))
 
     def set_dst_algorithm(self, algo):
-        """0: (N+1)-point FFT with symmetric pre/post-pass; 1: 2(N+1)-point odd-extension FFT."""
+        """Bit 0: 2(N+1)-point odd-extension FFT (else (N+1)-point with pre/post-pass); bit 1: 64-thread blocks."""
         _check(lib().hh_context_set_dst_algorithm(self.h, int(algo)))
 
     def set_memory_budget(self, nbytes):

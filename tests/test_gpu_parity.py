@@ -46,9 +46,9 @@ def relerr(a, b):
 
 
 # ------------------------------------------------------------------ DST
-@pytest.fixture(params=[0, 1], ids=["half", "oddext"])
+@pytest.fixture(params=[0, 1, 2, 3], ids=["half", "oddext", "half64", "oddext64"])
 def dctx(ctx, request):
-    """The context with DST algorithm 0 ((N+1)-point FFT) or 1 (odd extension)."""
+    """The context with each DST variant: (N+1)-point or odd-extension FFT, 256- or 64-thread blocks."""
     ctx.set_dst_algorithm(request.param)
     yield ctx
     ctx.set_dst_algorithm(0)
