@@ -124,7 +124,7 @@ def main():
         ctx.set_dst_algorithm(algo)
         _, per = time_tri(plan, rhs, out)
         print(f"DST algorithm {algo}: {1e3 * per[0]:9.1f} us per launch", flush=True)
-    ctx.set_dst_algorithm(0)
+    ctx.set_dst_algorithm(-1)
     if args.save:
         table = load_table()
         n = N

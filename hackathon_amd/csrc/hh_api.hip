@@ -49,7 +49,7 @@ struct hh_context {
     std::map<int, void *> tw64, tw32;   // twiddles by FFT length
     std::map<int, void *> S64, S32;     // dense DST matrices by N
     std::map<int, void *> th64, th32;   // half-length transform tables by N+1
-    int dst_algo = 0;                   // bit 0: odd extension (else (N+1)-point), bit 1: 64-thread blocks
+    int dst_algo = -1;                  // -1 auto; bit 0: odd extension (else (N+1)-point), bit 1: 64-thread blocks
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     double *d_norm = nullptr;
@@ -231,11 +231,21 @@ static int get_half_table(hh_context *c, int N, int f32, void **out) {
     return HH_OK;
 }
 
+// DST variant for lines of length N.  Automatic choice: the half-length
+// transform (measured 6% faster than the odd extension at N+1 = 1024) up to
+// N+1 = 1024; above, its odd outputs, prefix sums of M/2 transform values,
+// carry roundoff growing like sqrt(M) (2e-13 relative at N+1 = 4096), so the
+// odd-extension transform (error growing like log M) is used.
+static int dst_algo_for(const hh_context *c, int N) {
+    if (c->dst_algo >= 0) return c->dst_algo;
+    return N + 1 > 1024 ? 1 : 0;
+}
+
 // One DST pass over lines; in-place when the FFT path applies.
 static int dst_pass(hh_context *c, const hh::Lines &ln, int ti, int tc, int to, const void *in, void *out) {
     void *tw = nullptr, *S = nullptr;
     int rc;
-    const int algo = c->dst_algo;
+    const int algo = dst_algo_for(c, ln.N);
     if (hh::fft_supported(ln.N))
         rc = (algo & 1) == 0 ? get_half_table(c, ln.N, tc, &tw) : get_twiddles(c, ln.N, tc, &tw);
     else rc = get_dst_matrix(c, ln.N, tc, &S);
@@ -363,7 +373,7 @@ static int collect_timing(hh_context *c) {
 
 extern "C" int hh_context_set_dst_algorithm(hh_context *c, int algo) {
     if (!c) return fail(HH_ERR_ARG, "null context");
-    if (algo < 0 || algo > 3) return fail(HH_ERR_ARG, "DST algorithm must be 0..3 (got %d)", algo);
+    if (algo < -1 || algo > 3) return fail(HH_ERR_ARG, "DST algorithm must be -1..3 (got %d)", algo);
     std::lock_guard<std::mutex> lk(c->mu);
     c->dst_algo = algo;
     return HH_OK;
@@ -684,6 +694,10 @@ extern "C" int hh_plan_set_tri_shape(hh_plan *p, int W, int rows_per_thread) {
     if (P > n / 2) P = n / 2;
     if (P < 1 || W * P > 1024 || (n + P - 1) / P > rows_per_thread)
         return fail(HH_ERR_UNSUPPORTED, "tridiagonal shape W=%d rows=%d does not fit n=%d", W, rows_per_thread, n);
+    // the reduced system's two PCR buffers (A, C, D; FP64 complex) must fit the 160 KiB of LDS
+    if ((size_t)12 * P * W * 16 > (size_t)160 * 1024)
+        return fail(HH_ERR_UNSUPPORTED, "tridiagonal shape W=%d rows=%d needs more LDS than a CU has", W,
+                    rows_per_thread);
     p->tri_W = W;
     p->tri_mm = rows_per_thread;
     return HH_OK;
@@ -884,7 +898,7 @@ static int prepare_tables(hh_plan *p) {
         void *t = nullptr;
         int rc;
         if (hh::fft_supported(N))
-            rc = (c->dst_algo & 1) == 0 ? get_half_table(c, N, tc, &t) : get_twiddles(c, N, tc, &t);
+            rc = (dst_algo_for(c, N) & 1) == 0 ? get_half_table(c, N, tc, &t) : get_twiddles(c, N, tc, &t);
         else rc = get_dst_matrix(c, N, tc, &t);
         if (rc) return rc;
     }

@@ -184,12 +184,9 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
     // 1. downward sweep over rows s+1..e: row s+t becomes
     //    ab[t] x_s + x_{s+t} + cb[t] x_{s+t+1} = db[t]
     C2<T> pa = mk<T>(0, 0), pc = mk<T>(0, 0), pd = mk<T>(0, 0);
-    Band<T> bn = prob.band(md, min(s + 1, e));
 #pragma unroll
     for (int t = 1; t < MMAX; t++) {
-        // the next row's band is fetched one row ahead of its use
-        const Band<T> bq = bn;
-        if (t + 1 < MMAX) bn = prob.band(md, min(s + t + 1, e));
+        const Band<T> bq = prob.band(md, min(s + t, e));
         const C2<T> a = bq.sub, d = db[t];
         const C2<T> den = (t == 1) ? bq.dia : cfms<T>(a, pc, bq.dia);
         const C2<T> r = crcp<T>(den);
@@ -290,11 +287,10 @@ static hipError_t launch_tri_prob(const Prob &pb, int n, long nmodes, int batch,
     if (P < 1 || P > n / 2 || (n + P - 1) / P > MM || W * P > 1024) return hipErrorInvalidValue;
     int threads = ((W * P + 63) / 64) * 64;
     size_t shm = sizeof(C2<T>) * 2 * 3 * 2 * (size_t)P * W;   // two buffers of A, C, D
-    // 512-thread bound: 256 VGPRs per lane, no spills of the chunk registers
-    // up to 8 FP64 rows per thread; 16 FP64 rows need the 256-thread bound
-    // (one wave per SIMD, the whole register file) to stay out of scratch
+    // 512-thread bound: at most 256 VGPRs per lane and two waves per SIMD.
+    // (16 FP64 rows spill a few registers there; measured faster than the
+    // spill-free 256-thread bound, which leaves one wave per SIMD.)
     auto kfn = threads <= 512 ? k_tri<T, MM, 512, Prob> : k_tri<T, MM, 1024, Prob>;
-    if (sizeof(T) == 8 && MM >= 16 && threads <= 256) kfn = k_tri<T, MM, 256, Prob>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
@@ -380,6 +376,7 @@ hipError_t launch_compact(const Compact &cd, hipStream_t st) {
     switch (mm) {
     case 4: return launch_tri_prob<double, 4>(pb, cd.n, cd.nmodes, cd.batch, W, P, st);
     case 8: return launch_tri_prob<double, 8>(pb, cd.n, cd.nmodes, cd.batch, W, P, st);
+    case 16: return launch_tri_prob<double, 16>(pb, cd.n, cd.nmodes, cd.batch, W, P, st);
     default: return hipErrorInvalidValue;
     }
 }

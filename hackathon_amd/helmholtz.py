@@ -86,6 +86,7 @@ class Context:
         h = ctypes.c_void_p()
         _check(lib().hh_context_create(device, ctypes.byref(h)))
         self.h = h
+        self.algo = -1
         self.bind_stream()
 
     def bind_stream(self, stream=None):
@@ -98,8 +99,10 @@ class Context:
         _check(lib().hh_context_synchronize(self.h))
 
     def set_dst_algorithm(self, algo):
-        """Bit 0: 2(N+1)-point odd-extension FFT (else (N+1)-point with pre/post-pass); bit 1: 64-thread blocks."""
+        """-1: automatic by length; else bit 0: 2(N+1)-point odd-extension FFT (else (N+1)-point with
+        pre/post-pass), bit 1: 64-thread blocks."""
         _check(lib().hh_context_set_dst_algorithm(self.h, int(algo)))
+        self.algo = int(algo)
 
     def set_memory_budget(self, nbytes):
         _check(lib().hh_context_set_memory_budget(self.h, ctypes.c_size_t(int(nbytes))))

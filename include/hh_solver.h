@@ -58,10 +58,12 @@ size_t hh_context_bytes_in_use(const hh_context *ctx);
  * milliseconds, the number of launches and the algorithmic bytes moved
  * (each element read once and written once). */
 int hh_context_enable_timing(hh_context *ctx, int on);
-/* DST-I of power-of-two N+1.  Bit 0 of algo: 0 (default) = (N+1)-point FFT
- * of the symmetrised row plus a prefix-sum post-pass, 1 = 2(N+1)-point FFT
- * of the odd extension; bit 1: 64-thread blocks instead of 256.  All four
- * are orthonormal DST-I to roundoff. */
+/* DST-I of power-of-two N+1.  Bit 0 of algo: 0 = (N+1)-point FFT of the
+ * symmetrised row plus a prefix-sum post-pass, 1 = 2(N+1)-point FFT of the
+ * odd extension; bit 1: 64-thread blocks instead of 256.  All four are
+ * orthonormal DST-I to roundoff; the prefix sums of the first make its error
+ * grow like sqrt(N) rather than log N.  -1 (default): the (N+1)-point
+ * transform up to N+1 = 1024, the odd extension above. */
 int hh_context_set_dst_algorithm(hh_context *ctx, int algo);
 int hh_context_read_timing(hh_context *ctx, double ms[5], long launches[5], double bytes[5]);
 int hh_context_reset_timing(hh_context *ctx);

@@ -100,12 +100,15 @@ def test_set_operators_matches_new_plan(H, ctx, oracle):
     rhs = dev(np.stack([p["rhs"] for p in probs]))
     plan = H.Plan2D(ctx, N, N, ops)          # four operator slots
     first = plan.solve(rhs).cpu().numpy()
+    for b in range(4):
+        assert relerr(first[b].ravel(), probs[b]["sol"]) <= 1e-12
     plan.set_operators([ops[3], ops[2]])     # two operators: rhs b uses op b % 2
-    swapped = plan.solve(rhs).cpu().numpy()
-    fresh = H.Plan2D(ctx, N, N, [ops[3], ops[2]]).solve(rhs).cpu().numpy()
+    rhs2 = dev(np.stack([probs[3]["rhs"], probs[2]["rhs"]]))
+    swapped = plan.solve(rhs2).cpu().numpy()
+    fresh = H.Plan2D(ctx, N, N, [ops[3], ops[2]]).solve(rhs2).cpu().numpy()
     say("set_operators vs fresh plan")
     assert np.array_equal(swapped, fresh)
-    assert not np.array_equal(swapped, first)
-    assert relerr(swapped[1].ravel(), probs[2]["sol"]) <= 1e-12
+    assert np.array_equal(swapped[0], first[3]) and np.array_equal(swapped[1], first[2])
+    assert relerr(swapped[0].ravel(), probs[3]["sol"]) <= 1e-12
     with pytest.raises(H.HHError):
         plan.set_operators(ops + ops)        # more operators than slots

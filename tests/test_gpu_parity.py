@@ -51,7 +51,7 @@ def dctx(ctx, request):
     """The context with each DST variant: (N+1)-point or odd-extension FFT, 256- or 64-thread blocks."""
     ctx.set_dst_algorithm(request.param)
     yield ctx
-    ctx.set_dst_algorithm(0)
+    ctx.set_dst_algorithm(-1)
 
 
 def test_dst_golden(dctx):
@@ -65,22 +65,33 @@ def test_dst_golden(dctx):
         assert relerr(y, g["y_" + n]) <= 1e-13
 
 
+def dst_tol(algo, N):
+    """Relative max-norm bound of one DST-I.  The odd-extension transform's
+    roundoff grows like log N; the half-length transform's odd outputs are
+    prefix sums of N/2 transform values, so its roundoff grows like sqrt(N)
+    (measured 1.0e-13 after two transforms at N = 4095)."""
+    return 1e-13 if (algo & 1) or N < 1023 else 1e-13 * np.sqrt((N + 1) / 1024.0) * 2
+
+
 @pytest.mark.parametrize("N,rows", [(1, 5), (3, 7), (7, 300), (15, 33), (31, 129), (63, 64), (100, 37), (101, 3),
                                     (127, 1000), (200, 9), (255, 513), (511, 65), (1023, 257), (2047, 17),
                                     (4095, 5)])
 def test_dst_rows_vs_oracle(dctx, oracle, N, rows):
     ctx = dctx
-    say(f"dst rows N={N} rows={rows}")
+    tol = dst_tol(dctx.algo, N)
+    say(f"dst rows N={N} rows={rows} algo={dctx.algo}")
     rng = np.random.default_rng(N * 7 + rows)
     x = rng.standard_normal((rows, N)) + 1j * rng.standard_normal((rows, N))
     ref = oracle.dst1_rows(x)
     y = host(ctx.dst_rows(dev(x)))
-    assert relerr(y, ref) <= 1e-13
+    e1 = relerr(y, ref)
     # in place, and self-inverse
     t = dev(x)
     ctx.dst_rows(t, out=t)
     ctx.dst_rows(t, out=t)
-    assert relerr(host(t), x) <= 1e-13
+    e2 = relerr(host(t), x)
+    say(f"  err {e1:.2e}, twice {e2:.2e} (bound {tol:.1e})")
+    assert e1 <= tol and e2 <= tol
 
 
 @pytest.mark.parametrize("Nx,Ny,nl", [(15, 15, 3), (31, 63, 2), (63, 31, 4), (255, 255, 2), (20, 15, 3), (15, 24, 2),
@@ -197,7 +208,9 @@ def test_residual3d(H, ctx, oracle, order, N):
 
 # ------------------------------------------------------------------ RHS assembly
 @pytest.mark.parametrize("order,kind,p,N", [(2, 0, None, 31), (4, 0, None, 40), (6, 0, None, 63),
-                                            (6, 1, [50.0, 0.0, 7, 11, 1.3], 127), (6, 0, None, 2)])
+                                            (6, 1, [50.0, 0.0, 7, 11, 1.3], 127), (6, 0, None, 2),
+                                            (6, 1, [50.0, 0.0, 7, 11, 1.3], 1023), (6, 0, None, 600),
+                                            (6, 1, [30.0, 0.0, 3, 2, 0.2], 2047)])
 def test_rhs2d(H, ctx, oracle, order, kind, p, N):
     say(f"rhs2d order={order} kind={kind} N={N}")
     s = oracle.setup2d(order, N, N, kind, p)
@@ -211,7 +224,8 @@ def test_rhs2d(H, ctx, oracle, order, kind, p, N):
 
 
 @pytest.mark.parametrize("order,kind,p,N", [(2, 0, [10, 9, 10, 9], 15), (4, 0, [10, 9, 10, 9], 20),
-                                            (6, 0, [10, 9, 10, 9], 31), (6, 1, [30.0, 0.0, 3, 2, 1, 1.0, 1.0], 24)])
+                                            (6, 0, [10, 9, 10, 9], 31), (6, 1, [30.0, 0.0, 3, 2, 1, 1.0, 1.0], 24),
+                                            (6, 1, [30.0, 0.0, 3, 2, 1, 1.0, 1.0], 127)])
 def test_rhs3d(H, ctx, oracle, order, kind, p, N):
     say(f"rhs3d order={order} kind={kind} N={N}")
     s = oracle.setup3d(order, N, N, N, kind, p)
