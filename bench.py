@@ -201,7 +201,7 @@ def run_3d(args, torch, H):
         Fh = F.cpu().numpy().ravel()
         threads = max(1, min(len(os.sched_getaffinity(0)), 16))
         t0 = time.perf_counter()
-        LUU = O.lu3d(N, N, N, coef)
+        LUU = O.lu3d(N, N, N, np.concatenate(coef))
         ref = O.solver3d(N, N, N, Fh, LUU, threads)
         t_cpu = time.perf_counter() - t0
         for name, out in (("fp64", s64), ("mixed", smx)):
