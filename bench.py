@@ -151,7 +151,7 @@ def run_3d(args, torch, H):
     dev = torch.device("cuda:0")
     N, kk, order = 255 if args.N == 1023 else args.N, 30.0, 6
     h = 1.0 / (N + 1)
-    rng = np.random.default_rng(args.seed)
+    rng = np.random.default_rng((args.seed, 3))
     abc = [int(v) for v in rng.integers(1, 21, size=3)]
     ctx = H.Context(0)
     k2e = np.full(N + 2, kk * kk, np.complex128)
@@ -202,22 +202,31 @@ def run_3d(args, torch, H):
         threads = max(1, min(len(os.sched_getaffinity(0)), 16))
         t0 = time.perf_counter()
         LUU = O.lu3d(N, N, N, np.concatenate(coef))
+        t_lu = time.perf_counter() - t0
         ref = O.solver3d(N, N, N, Fh, LUU, threads)
-        t_cpu = time.perf_counter() - t0
         for name, out in (("fp64", s64), ("mixed", smx)):
             g = out.cpu().numpy().ravel()
             res[name]["max_rel_err_vs_oracle"] = float(np.abs(g - ref).max() / np.abs(ref).max())
         if not args.no_cpu:
-            cpu = {"value": N ** 3 / t_cpu, "unit": "complex DOF/s", "cores": threads, "kind": "port",
-                   "sample": f"oracle LU + solver() of the 3D reference CPU path (OpenMP over layers/modes) on one "
-                             f"right-hand side of the same {N}^3 k=30 problem, {t_cpu:.1f} s"}
-    mixed_ok = res["mixed"].get("max_rel_err_vs_oracle", 0.0) <= 1e-10
-    value = res["mixed"]["dof_per_s"] if mixed_ok else res["fp64"]["dof_per_s"]
-    return _dof_line(" (3D)", value, "complex DOF/s", args.steps, args.warmup, res["mixed"]["ms_per_solve"],
+            done, t_used = 0, 0.0
+            while t_used < args.cpu_seconds:   # solver() only, as the reference times it
+                t0 = time.perf_counter()
+                O.solver3d(N, N, N, Fh, LUU, threads)
+                t_used += time.perf_counter() - t0
+                done += 1
+            cpu = {"value": done * N ** 3 / t_used, "unit": "complex DOF/s", "cores": threads, "kind": "port",
+                   "sample": f"oracle solver() of the 3D reference CPU path (OpenMP over layers/modes), {done} solves "
+                             f"of the same {N}^3 k=30 right-hand side, {t_used:.1f} s (factorisation LU() "
+                             f"{t_lu:.1f} s, untimed like the reference)"}
+    ok = {k: res[k].get("max_rel_err_vs_oracle", 0.0) <= 1e-10 for k in ("fp64", "mixed")}
+    best = max((k for k in ("fp64", "mixed") if ok[k]), key=lambda k: res[k]["dof_per_s"], default="fp64")
+    return _dof_line(" (3D)", res[best]["dof_per_s"], "complex DOF/s", args.steps, args.warmup,
+                     res[best]["ms_per_solve"],
                      {"workload": f"3D compact Helmholtz N={N}^3, k=30, order 6, u=(1+i) sin(a pi x) sin(b pi y) "
-                                  f"sin(c pi z), (a,b,c)={tuple(abc)} from seed {args.seed}, GPU RHS assembly",
+                                  f"sin(c pi z), (a,b,c)={tuple(abc)} from seed ({args.seed}, 3), GPU RHS assembly",
                       "grid": [N, N, N], "orders": [order]},
-                     dtype="f32+f64" if mixed_ok else "f64", value_path="mixed" if mixed_ok else "fp64",
+                     dtype="f32+f64" if best == "mixed" else "f64", value_path=best,
+                     p50_ms=res[best]["p50_ms"], p99_ms=res[best]["p99_ms"],
                      mixed_over_fp64=res["mixed"]["dof_per_s"] / res["fp64"]["dof_per_s"], results=res,
                      cpu_baseline=cpu)
 

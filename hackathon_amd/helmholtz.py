@@ -294,8 +294,15 @@ class Plan2D(Plan):
         _require_cuda(u_e, "u_e")
         batch = f_e.numel() // ((self.Nx + 2) * (self.Ny + 2))
         out = f_e.new_empty((batch, self.Ny, self.Nx)) if out is None else out
-        k2e = _c128(k2e, self.Ny + 2)
-        k2yb = _c128(np.zeros(2) if k2yb is None else k2yb, 2)
+        # one k^2 profile (and (k^2)_y pair) per operator; a single one is shared
+        k2e = _c128(k2e)
+        if k2e.size == self.Ny + 2:
+            k2e = np.tile(k2e, self.nop)
+        k2e = _c128(k2e, (self.Ny + 2) * self.nop)
+        k2yb = _c128(np.zeros(2) if k2yb is None else k2yb)
+        if k2yb.size == 2:
+            k2yb = np.tile(k2yb, self.nop)
+        k2yb = _c128(k2yb, 2 * self.nop)
         null = ctypes.c_void_p(0)
         dp = lambda t: _dptr(t) if t is not None else null
         self.ctx.bind_stream()
@@ -329,8 +336,14 @@ class Plan3D(Plan):
         _require_cuda(u_e, "u_e")
         batch = f_e.numel() // ((self.Nx + 2) * (self.Ny + 2) * (self.Nz + 2))
         out = f_e.new_empty((batch, self.Nz, self.Ny, self.Nx)) if out is None else out
-        k2e = _c128(k2e, self.Nz + 2)
-        k2d = _c128(np.zeros(4) if k2d is None else k2d, 4)
+        k2e = _c128(k2e)
+        if k2e.size == self.Nz + 2:
+            k2e = np.tile(k2e, self.nop)
+        k2e = _c128(k2e, (self.Nz + 2) * self.nop)
+        k2d = _c128(np.zeros(4) if k2d is None else k2d)
+        if k2d.size == 4:
+            k2d = np.tile(k2d, self.nop)
+        k2d = _c128(k2d, 4 * self.nop)
         null = ctypes.c_void_p(0)
         dp = lambda t: _dptr(t) if t is not None else null
         self.ctx.bind_stream()
