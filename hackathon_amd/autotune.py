@@ -22,7 +22,7 @@ import numpy as np
 from . import helmholtz as H
 
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
-CANDIDATES = [(w, m) for m in (4, 8, 16) for w in (1, 2, 4, 8, 16, 32)]
+CANDIDATES = [(w, m, r) for m in (8, 16) for w in (1, 2, 4, 8, 16) for r in (1, 2)]
 
 
 def _bucket(batch):
@@ -49,7 +49,8 @@ def save_table(table, path=TABLE):
 
 
 def time_tri(plan, rhs, out, reps=3):
-    """Average tridiagonal-kernel milliseconds of one solve of rhs."""
+    """(kernel microseconds of one solve of rhs, average milliseconds per
+    launch of each kernel category), from the library's event timers."""
     lib = H.lib()
     ctx = plan.ctx
     plan.solve(rhs, out=out)  # warm-up (twiddles, scratch, code objects)
@@ -61,25 +62,40 @@ def time_tri(plan, rhs, out, reps=3):
     ms = (ctypes.c_double * 5)()
     nl = (ctypes.c_long * 5)()
     lib.hh_context_read_timing(ctx.h, ms, nl, None)
-    return ms[2] / max(nl[2], 1), [ms[k] / max(nl[k], 1) for k in range(5)]
+    return 1e3 * sum(ms) / reps, [ms[k] / max(nl[k], 1) for k in range(5)]
 
 
 def tune(plan, rhs, out, candidates=CANDIDATES, reps=3, verbose=False):
-    """Try every shape that fits; leave the fastest set on the plan."""
+    """Try every shape (and, in 2D, spectrum layout) that fits; leave the
+    fastest, by kernel time of the whole solve, set on the plan."""
     lib = H.lib()
     results = []
-    for w, m in candidates:
-        if lib.hh_plan_set_tri_shape(plan.h, w, m) != 0:
+    batch = plan._batch_of(rhs)
+    tiles = (0, -1, 8, 16, 32) if plan.dim == 2 else (-1,)
+    for w, m, r in candidates:
+        if r > 1 and batch < 2 * r:
             continue
-        t, _ = time_tri(plan, rhs, out, reps)
-        results.append((t, w, m))
-        if verbose:
-            print(f"  W={w:3d} rows={m:3d}: {1e3 * t:9.1f} us", flush=True)
+        if lib.hh_plan_set_tri_shape(plan.h, w, m, r) != 0:
+            continue
+        for tw in tiles:
+            if tw > 0 and (tw < w or tw % w):
+                continue
+            H._check(lib.hh_plan_set_tile_width(plan.h, tw))
+            try:
+                t, per = time_tri(plan, rhs, out, reps)
+            except H.HHError:
+                continue
+            results.append((t, w, m, r, tw, per[0] * 1e3, per[2] * 1e3))
+            if verbose:
+                print(f"  W={w:3d} rows={m:3d} rhs={r} tile={tw:3d}: solve {t:9.1f} us "
+                      f"(DST {1e3 * per[0]:7.1f}, tridiagonal {1e3 * per[2]:7.1f} us per launch)", flush=True)
     results.sort()
     best = results[0]
-    lib.hh_plan_set_tri_shape(plan.h, best[1], best[2])
-    return {"W": best[1], "rows": best[2], "us": 1e3 * best[0],
-            "all": [{"W": w, "rows": m, "us": 1e3 * t} for t, w, m in results]}
+    lib.hh_plan_set_tri_shape(plan.h, best[1], best[2], best[3])
+    lib.hh_plan_set_tile_width(plan.h, best[4])
+    return {"W": best[1], "rows": best[2], "rhs": best[3], "tile": best[4], "us": best[0],
+            "all": [{"W": w, "rows": m, "rhs": r, "tile": tw, "us": t, "dst_us": d, "tri_us": q}
+                    for t, w, m, r, tw, d, q in results]}
 
 
 def apply_tuned(plan, batch, table=None):
@@ -89,7 +105,8 @@ def apply_tuned(plan, batch, table=None):
     nmodes = plan.Nx if plan.dim == 2 else plan.Nx * plan.Ny
     e = table.get(key(plan.dim, n, nmodes, batch, plan.precision))
     if e:
-        H._check(H.lib().hh_plan_set_tri_shape(plan.h, e["W"], e["rows"]))
+        H._check(H.lib().hh_plan_set_tri_shape(plan.h, e["W"], e["rows"], e.get("rhs", 1)))
+        H._check(H.lib().hh_plan_set_tile_width(plan.h, e.get("tile", -1)))
     return e
 
 
@@ -130,6 +147,7 @@ def main():
         n = N
         nmodes = N if args.dim == 2 else N * N
         table[key(args.dim, n, nmodes, args.batch, args.precision)] = {"W": res["W"], "rows": res["rows"],
+                                                                       "rhs": res["rhs"], "tile": res["tile"],
                                                                        "us": round(res["us"], 1)}
         save_table(table)
 

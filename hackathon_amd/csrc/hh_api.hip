@@ -102,6 +102,11 @@ struct hh_plan {
     double tol = 1e-12;
     int max_iter = 6;
     int tri_W = 0, tri_mm = 0;                   // tuned tridiagonal block shape (0 = heuristic)
+    int tri_nr = 1;                              // right-hand sides per thread of the tuned shape
+    // 2D spectrum layout: 0 row-major (default: measured fastest for the whole
+    // solve, 1753 us against 1761-1844 us for tiles of 4..32 modes at N=1023,
+    // 64 right-hand sides), -1 tiles of the block width, else the tile width
+    int tile_w = 0;
     int last_sweeps = 0;
     double last_rel = 0;
 };
@@ -673,6 +678,7 @@ extern "C" size_t hh_plan_workspace_bytes(const hh_plan *p, int batch) {
     size_t f = (size_t)hh_plan_field_size(p) * batch;
     size_t ws = 0;
     if (!plan_all_fft(p)) ws += f * 16;      // ping-pong buffer of the dense DST path
+    else if (p->dim == 2 && p->tile_w != 0) ws += f * (p->prec == HH_PREC_MIXED ? 8 : 16);   // mode-tiled spectrum
     if (p->prec == HH_PREC_MIXED) ws += f * 8;   // FP32 correction
     return ws;
 }
@@ -684,40 +690,55 @@ extern "C" int hh_plan_set_refine(hh_plan *p, double tol, int max_iter) {
     return HH_OK;
 }
 
-extern "C" int hh_plan_set_tri_shape(hh_plan *p, int W, int rows_per_thread) {
+extern "C" int hh_plan_set_tri_shape(hh_plan *p, int W, int rows_per_thread, int rhs_per_thread) {
     if (!p) return fail(HH_ERR_ARG, "null plan");
-    if (W == 0 && rows_per_thread == 0) { p->tri_W = p->tri_mm = 0; return HH_OK; }
+    if (W == 0 && rows_per_thread == 0) { p->tri_W = p->tri_mm = 0; p->tri_nr = 1; return HH_OK; }
     int n = p->dim == 2 ? p->Ny : p->Nz;
-    if (W < 1 || W > 64 || (rows_per_thread != 4 && rows_per_thread != 8 && rows_per_thread != 16))
-        return fail(HH_ERR_ARG, "tridiagonal shape W=%d rows=%d not available", W, rows_per_thread);
+    if (W < 1 || W > 64 || (rows_per_thread != 4 && rows_per_thread != 8 && rows_per_thread != 16) ||
+        (rhs_per_thread != 1 && rhs_per_thread != 2 && rhs_per_thread != 4))
+        return fail(HH_ERR_ARG, "tridiagonal shape W=%d rows=%d rhs=%d not available", W, rows_per_thread,
+                    rhs_per_thread);
     int P = (n + rows_per_thread - 1) / rows_per_thread;
     if (P > n / 2) P = n / 2;
     if (P < 1 || W * P > 1024 || (n + P - 1) / P > rows_per_thread)
         return fail(HH_ERR_UNSUPPORTED, "tridiagonal shape W=%d rows=%d does not fit n=%d", W, rows_per_thread, n);
-    // the reduced system's two PCR buffers (A, C, D; FP64 complex) must fit the 160 KiB of LDS
-    if ((size_t)12 * P * W * 16 > (size_t)160 * 1024)
-        return fail(HH_ERR_UNSUPPORTED, "tridiagonal shape W=%d rows=%d needs more LDS than a CU has", W,
-                    rows_per_thread);
+    // the reduced system's two PCR buffers (A, C and the right-hand sides; FP64 complex) must fit the
+    // 160 KiB of LDS
+    if ((size_t)2 * (2 + rhs_per_thread) * 2 * P * W * 16 > (size_t)160 * 1024)
+        return fail(HH_ERR_UNSUPPORTED, "tridiagonal shape W=%d rows=%d rhs=%d needs more LDS than a CU has", W,
+                    rows_per_thread, rhs_per_thread);
     p->tri_W = W;
     p->tri_mm = rows_per_thread;
+    p->tri_nr = rhs_per_thread;
     return HH_OK;
 }
 
-extern "C" int hh_plan_get_tri_shape(const hh_plan *p, int *W, int *P, int *rows_per_thread) {
+extern "C" int hh_plan_set_tile_width(hh_plan *p, int tw) {
+    if (!p) return fail(HH_ERR_ARG, "null plan");
+    if (tw != -1 && (tw < 0 || tw > 64 || (tw & (tw - 1))))
+        return fail(HH_ERR_ARG, "tile width must be -1, 0 or a power of two up to 64 (got %d)", tw);
+    p->tile_w = tw;
+    return HH_OK;
+}
+
+extern "C" int hh_plan_get_tri_shape(const hh_plan *p, int *W, int *P, int *rows_per_thread, int *rhs_per_thread) {
     if (!p) return fail(HH_ERR_ARG, "null plan");
     hh::TriHelm t;
     int n = p->dim == 2 ? p->Ny : p->Nz;
     long nmodes = p->dim == 2 ? p->Nx : (long)p->Nx * p->Ny;
-    hh::tri_shape(n, nmodes, p->prec == HH_PREC_F64 ? hh::F64 : hh::F64, &t.W, &t.P, &t.mmax);
+    hh::tri_shape(n, nmodes, hh::F64, &t.W, &t.P, &t.mmax);
+    t.nr = 1;
     if (p->tri_W > 0) {
         t.W = p->tri_W;
         t.mmax = p->tri_mm;
+        t.nr = p->tri_nr;
         t.P = (n + t.mmax - 1) / t.mmax;
         if (t.P > n / 2) t.P = n / 2;
     }
     if (W) *W = t.W;
     if (P) *P = t.P;
     if (rows_per_thread) *rows_per_thread = t.mmax;
+    if (rhs_per_thread) *rhs_per_thread = t.nr;
     return HH_OK;
 }
 
@@ -751,9 +772,11 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
     if (tri_comp == 0 && tc == 1) {   // FP64 elimination on FP32 storage: FP64 register budget
         hh::tri_shape(n, nmodes, hh::F64, &t.W, &t.P, &t.mmax);
     }
+    t.nr = 1;
     if (p->tri_W > 0 && p->tri_mm > 0) {   // shape chosen by the autotuner
         t.W = p->tri_W;
         t.mmax = p->tri_mm;
+        t.nr = tri_comp == 0 ? p->tri_nr : 1;
         t.P = (n + t.mmax - 1) / t.mmax;
         if (t.P > n / 2) t.P = n / 2;
     }
@@ -765,8 +788,23 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
     };
     const bool fft = plan_all_fft(p);
     if (p->dim == 2) {
+        // FFT path: the forward row DST writes the spectrum mode-tiled into
+        // ws (tiles of W modes, the tridiagonal block width), the mode solves
+        // run there in place, the inverse row DST reads it back row-major
+        // into sol.  Dense path: ws is a plain row-major intermediate.
         hh::Lines lx = rows_lines(Nx, (long)Ny * batch);
-        // intermediate buffer: sol (in place) on the FFT path; ws otherwise
+        const int tw = p->tile_w < 0 ? t.W : p->tile_w;
+        if (fft && tw > 0) {
+            if (!ws || ws == rhs || ws == sol) return fail(HH_ERR_ARG, "internal: missing workspace");
+            t.tiled = tw;
+            hh::Lines fwd = lx, inv = lx;
+            fwd.out_tw = tw; fwd.ny = Ny; fwd.field = t.field;
+            inv.in_tw = tw; inv.ny = Ny; inv.field = t.field;
+            if ((rc = dst_pass(c, fwd, ti, tc, tc, rhs, ws))) return rc;
+            if ((rc = tri(ws))) return rc;
+            return dst_pass(c, inv, tc, tc, to, ws, sol);
+        }
+        // row-major spectrum: in sol (in place) on the FFT path, in ws otherwise
         void *mid = fft ? (tc == to ? sol : ws) : ws;
         if (!mid) return fail(HH_ERR_ARG, "internal: missing workspace");
         if ((rc = dst_pass(c, lx, ti, tc, tc, rhs, mid))) return rc;
@@ -826,21 +864,23 @@ static int solve_locked(hh_plan *p, const void *rhs, void *sol, int batch) {
     if (ws_bytes && (rc = ensure_scratch(c, ws_bytes))) return rc;
     const long n = hh_plan_field_size(p) * batch;
     if (p->prec == HH_PREC_F64) {
-        void *ws = plan_all_fft(p) ? nullptr : c->scratch;
+        void *ws = (plan_all_fft(p) && (p->dim == 3 || p->tile_w == 0)) ? nullptr : c->scratch;
         return solve_core(p, 0, 0, 0, 0, rhs, sol, ws, batch);
     }
     // mixed: FP32 transforms and storage, FP64 elimination, FP64 residual
-    // and update; the correction lives in FP32 scratch
+    // and update; the correction lives in FP32 scratch, followed (2D) by the
+    // FP32 mode-tiled spectrum of the solves
     void *dlt = c->scratch;
+    void *tws = (p->dim == 2 && p->tile_w != 0) ? (void *)((char *)c->scratch + (size_t)n * 8) : dlt;
     if (!plan_all_fft(p)) return fail(HH_ERR_UNSUPPORTED, "mixed precision needs power-of-two N+1");
     if (rhs == sol) return fail(HH_ERR_ARG, "mixed precision solve needs d_sol != d_rhs");
-    if ((rc = solve_core(p, 0, 1, 0, 0, rhs, sol, dlt, batch))) return rc;   // u0 = A32^{-1} F
+    if ((rc = solve_core(p, 0, 1, 0, 0, rhs, sol, tws, batch))) return rc;   // u0 = A32^{-1} F
     if (p->tol == 0) {
         // fixed number of sweeps: no norms, no host synchronisation, so the
         // whole solve can be recorded into a HIP graph
         for (int it = 0; it < p->max_iter; it++) {
             if ((rc = residual_into(p, sol, rhs, dlt, 1, batch))) return rc;
-            if ((rc = solve_core(p, 1, 1, 1, 0, dlt, dlt, dlt, batch))) return rc;
+            if ((rc = solve_core(p, 1, 1, 1, 0, dlt, dlt, tws, batch))) return rc;
             TimeScope ts(c, 3, (double)n * 40);
             hipError_t e = hh::launch_axpy_update(n, sol, dlt, 1, -1.0, c->stream);
             if (e != hipSuccess) return fail(HH_ERR_DEVICE, "update launch failed: %s", hipGetErrorString(e));
@@ -861,7 +901,7 @@ static int solve_locked(hh_plan *p, const void *rhs, void *sol, int batch) {
         if ((rc = norms_sync(c, n, dlt, 1, nullptr, nr))) return rc;
         rel = nr[0] / fn;
         if (rel <= p->tol || sweeps >= p->max_iter) break;
-        if ((rc = solve_core(p, 1, 1, 1, 0, dlt, dlt, dlt, batch))) return rc;   // A d = r
+        if ((rc = solve_core(p, 1, 1, 1, 0, dlt, dlt, tws, batch))) return rc;   // A d = r
         TimeScope ts(c, 3, (double)n * 40);
         hipError_t e = hh::launch_axpy_update(n, sol, dlt, 1, -1.0, c->stream);    // u -= d
         if (e != hipSuccess) return fail(HH_ERR_DEVICE, "update launch failed: %s", hipGetErrorString(e));

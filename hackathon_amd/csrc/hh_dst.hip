@@ -39,10 +39,84 @@ template <int L, int TPB = 256> struct dst_cfg {
     static constexpr int LP = fft_shape<L>::LP;
 };
 
-template <typename Ti, typename T, typename To, int L, int TPB>
+// Position of element k of line `row` (the lines of every right-hand side
+// stacked, Ny lines each) in a buffer that is row-major (tw = 0) or
+// mode-tiled (tw > 0, a power of two): the N modes of a right-hand side are
+// cut into tiles of tw neighbouring modes, tile t0 holds its modes for all
+// Ny lines contiguously, [t0 * Ny + j * width + (k - t0)], width = min(tw,
+// N - t0).  The tiled form is what the tridiagonal pass reads: each of its
+// threads walks down a tile in whole width*16-byte segments.
+struct LinePos {
+    long base;
+    int j, N, Ny, tw;
+    HH_DEV long at(int k) const {
+        if (tw == 0) return base + k;
+        const int t0 = k & ~(tw - 1);
+        const int w = min(tw, N - t0);
+        return base + (long)t0 * Ny + (long)j * w + (k - t0);
+    }
+};
+struct RowMap {
+    int N, Ny, tw;
+    long field;
+    HH_DEV LinePos line(long row) const {
+        LinePos p;
+        p.N = N; p.Ny = Ny; p.tw = tw;
+        if (tw == 0) { p.base = row * N; p.j = 0; return p; }
+        const long b = row / Ny;
+        p.j = (int)(row - b * Ny);
+        p.base = b * field;
+        return p;
+    }
+    // the line `l` rows after a known one (l small: steps over right-hand sides by subtraction)
+    HH_DEV LinePos next(LinePos p, int l) const {
+        if (tw == 0) { p.base += (long)l * N; return p; }
+        int j = p.j + l;
+        while (j >= Ny) { j -= Ny; p.base += field; }
+        p.j = j;
+        return p;
+    }
+};
+
+// Coalesced load of LPB consecutive lines from the mode-tiled layout into
+// the LDS line buffers (element k of line l at position k+1), in tile-major
+// order so that each tile's LPB x tw elements are read as one contiguous
+// run.  Every load of a thread is issued before the first LDS write.
+template <typename Ti, typename T, int N, int LPB, int LP, int NT>
+HH_DEV void load_tiled_rows(T *smem, const C2<Ti> *in, const RowMap &map, long row0, int nr) {
+    constexpr int LGL = LPB == 1 ? 0 : LPB == 2 ? 1 : LPB == 4 ? 2 : LPB == 8 ? 3 : LPB == 16 ? 4
+                      : LPB == 32 ? 5 : LPB == 64 ? 6 : LPB == 128 ? 7 : 8;
+    static_assert((1 << LGL) == LPB, "lines per block must be a power of two");
+    constexpr int ITER = ((N + 63) * LPB + NT - 1) / NT;   // tiles are at most 64 wide
+    const int lg = 31 - __clz(map.tw);
+    const int tw = 1 << lg;
+    const int slots = ((N + tw - 1) >> lg) << (lg + LGL);
+    const LinePos l0 = map.line(row0);
+    C2<T> v[ITER];
+    int pos[ITER];
+#pragma unroll
+    for (int i = 0; i < ITER; i++) {
+        const int e = threadIdx.x + i * NT;
+        const int tile = e >> (lg + LGL);
+        const int rem = e & ((1 << (lg + LGL)) - 1);
+        const int l = rem >> lg, k = (tile << lg) + (rem & (tw - 1));
+        const bool ok = e < slots && k < N && l < nr;
+        v[i] = cconv<T, Ti>(in[ok ? map.next(l0, l).at(k) : l0.at(0)]);
+        pos[i] = ok ? (2 * l) * LP + lds_pad(k + 1) : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < ITER; i++) {
+        if (pos[i] >= 0) {
+            smem[pos[i]] = v[i].x;
+            smem[pos[i] + LP] = v[i].y;
+        }
+    }
+}
+
+template <typename Ti, typename T, typename To, int L, int TPB, bool TIN>
 __global__ __launch_bounds__((dst_cfg<L, TPB>::THREADS)) void k_dst_fft_rows(
     const C2<Ti> *in, C2<To> *out, long nrows, T scale,
-    const C2<T> *__restrict__ tw) {
+    const C2<T> *__restrict__ tw, RowMap min_, RowMap mout) {
     constexpr int P = dst_cfg<L, TPB>::P, LPB = dst_cfg<L, TPB>::LPB, LP = dst_cfg<L, TPB>::LP;
     constexpr int M = L / 2, N = M - 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -51,20 +125,28 @@ __global__ __launch_bounds__((dst_cfg<L, TPB>::THREADS)) void k_dst_fft_rows(
     T *sre = smem + (2 * li) * LP, *sim = smem + (2 * li + 1) * LP;
     const long row = (long)blockIdx.x * LPB + li;
     const bool valid = row < nrows;
-    const C2<Ti> *x = in + (valid ? row : 0) * N;
-    C2<To> *y = out + (valid ? row : 0) * N;
+    const LinePos lout = mout.line(valid ? row : 0);
+    if constexpr (TIN) {   // tiled input: stage the block's lines through LDS
+        constexpr int NT = dst_cfg<L, TPB>::THREADS;
+        const long row0 = (long)blockIdx.x * LPB;
+        const long nr = nrows - row0 < LPB ? nrows - row0 : LPB;
+        load_tiled_rows<Ti, T, N, LPB, LP, NT>(smem, in, min_, row0, (int)nr);
+        __syncthreads();
+    }
     // Branch-free: every load is issued (index clamped into the row) and
     // the zero entries of the odd extension are selected afterwards, so the
     // 16 loads of a thread are all in flight together.
     auto get = [&](int n) -> C2<T> {
         const bool zero = (n == 0) || (n == M);
         const int i = zero ? 0 : (n < M ? n - 1 : L - n - 1);
-        const C2<T> v = cconv<T, Ti>(x[i]);
+        C2<T> v;
+        if constexpr (TIN) v = mk<T>(sre[lds_pad(i + 1)], sim[lds_pad(i + 1)]);
+        else v = cconv<T, Ti>(in[(valid ? row : 0) * N + i]);   // row-major input
         const C2<T> r = n < M ? v : cneg<T>(v);
         return zero ? mk<T>(0, 0) : r;
     };
     auto put = [&](int k, C2<T> z) {
-        if (valid && k >= 1 && k <= N) y[k - 1] = cconv<To, T>(mk<T>(-z.y * scale, z.x * scale));
+        if (valid && k >= 1 && k <= N) out[lout.at(k - 1)] = cconv<To, T>(mk<T>(-z.y * scale, z.x * scale));
     };
     fft_line<T, L, false>(sre, sim, tw, t, get, put);
 }
@@ -221,10 +303,11 @@ HH_DEV void half_post(T *sre, T *sim, T *tot_re, T *tot_im, int t, T scale) {
 }
 
 // rows (contiguous lines); aux = [twiddles of length M | sin(pi j/M), j < M]
-template <typename Ti, typename T, typename To, int M, int TPB>
+template <typename Ti, typename T, typename To, int M, int TPB, bool TIN>
 __global__ __launch_bounds__((half_cfg<M, TPB>::THREADS)) void k_dst_half_rows(const C2<Ti> *in, C2<To> *out,
                                                                             long nrows, T scale,
-                                                                            const C2<T> *__restrict__ aux) {
+                                                                            const C2<T> *__restrict__ aux,
+                                                                            RowMap min_, RowMap mout) {
     using cfg = half_cfg<M, TPB>;
     constexpr int P = cfg::P, LPB = cfg::LPB, LP = cfg::LP, N = M - 1, NT = cfg::THREADS;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -235,13 +318,25 @@ __global__ __launch_bounds__((half_cfg<M, TPB>::THREADS)) void k_dst_half_rows(c
     const long row0 = (long)blockIdx.x * LPB;
     const long row = row0 + li;
     const bool valid = row < nrows;
-    const C2<Ti> *x = in + (valid ? row : 0) * N;
     const T *sn = reinterpret_cast<const T *>(aux + M);
+    const long nr = nrows - row0 < LPB ? nrows - row0 : LPB;
+    if constexpr (TIN) {   // tiled input: stage the block's lines through LDS
+        load_tiled_rows<Ti, T, N, LPB, LP, NT>(smem, in, min_, row0, (int)nr);
+        __syncthreads();
+    }
     // branch-free (clamped indices, zero selected for n = 0) so that all
     // loads of a thread are in flight together; rows past the end read row 0
     auto get = [&](int n) -> C2<T> {
         const int ia = n > 0 ? n - 1 : 0, ib = n > 0 ? M - n - 1 : 0;
-        const C2<T> a = cconv<T, Ti>(x[ia]), b = cconv<T, Ti>(x[ib]);
+        C2<T> a, b;
+        if constexpr (TIN) {
+            a = mk<T>(sre[lds_pad(ia + 1)], sim[lds_pad(ia + 1)]);
+            b = mk<T>(sre[lds_pad(ib + 1)], sim[lds_pad(ib + 1)]);
+        } else {   // row-major input
+            const C2<Ti> *x = in + (valid ? row : 0) * N;
+            a = cconv<T, Ti>(x[ia]);
+            b = cconv<T, Ti>(x[ib]);
+        }
         const T s = sn[n];
         const C2<T> r = mk<T>(s * (a.x + b.x) + (T)0.5 * (a.x - b.x), s * (a.y + b.y) + (T)0.5 * (a.y - b.y));
         return n > 0 ? r : mk<T>(0, 0);
@@ -255,15 +350,34 @@ __global__ __launch_bounds__((half_cfg<M, TPB>::THREADS)) void k_dst_half_rows(c
     constexpr int NW = P > 64 ? P / 64 : 1;
     half_post<T, M>(sre, sim, tot + li * 2 * NW, tot + li * 2 * NW + NW, t, scale);
     // coalesced store of the block's rows (contiguous in memory)
-    const long nr = nrows - row0 < LPB ? nrows - row0 : LPB;
-    C2<To> *y = out + row0 * N;
     constexpr int ITER = (LPB * N + NT - 1) / NT;
+    if (mout.tw == 0) {
+        C2<To> *y = out + row0 * N;
 #pragma unroll
-    for (int i = 0; i < ITER; i++) {
-        const int e = threadIdx.x + i * NT;
-        if (e < nr * N) {
-            const int l = e / N, m = e % N + 1;
-            y[e] = cconv<To, T>(mk<T>(smem[(2 * l) * LP + lds_pad(m)], smem[(2 * l + 1) * LP + lds_pad(m)]));
+        for (int i = 0; i < ITER; i++) {
+            const int e = threadIdx.x + i * NT;
+            if (e < nr * N) {
+                const int l = e / N, m = e % N + 1;
+                y[e] = cconv<To, T>(mk<T>(smem[(2 * l) * LP + lds_pad(m)], smem[(2 * l + 1) * LP + lds_pad(m)]));
+            }
+        }
+    } else {
+        // tile-major order: the block's lines of one tile are contiguous there
+        const int lg = 31 - __clz(mout.tw);   // tw is a power of two, and so is LPB
+        constexpr int LGL = LPB == 1 ? 0 : LPB == 2 ? 1 : LPB == 4 ? 2 : LPB == 8 ? 3 : LPB == 16 ? 4
+                          : LPB == 32 ? 5 : LPB == 64 ? 6 : LPB == 128 ? 7 : 8;
+        static_assert((1 << LGL) == LPB, "lines per block must be a power of two");
+        const int tw = 1 << lg;
+        const int slots = ((N + tw - 1) >> lg) << (lg + LGL);
+        // the block's lines, each with its position in the tiled field
+        const LinePos l0 = mout.line(row0);
+        for (int e = threadIdx.x; e < slots; e += NT) {
+            const int tile = e >> (lg + LGL);
+            const int rem = e & ((1 << (lg + LGL)) - 1);
+            const int l = rem >> lg, k = (tile << lg) + (rem & (tw - 1));
+            if (k < N && l < nr)
+                out[mout.next(l0, l).at(k)] =
+                    cconv<To, T>(mk<T>(smem[(2 * l) * LP + lds_pad(k + 1)], smem[(2 * l + 1) * LP + lds_pad(k + 1)]));
         }
     }
 }
@@ -380,13 +494,15 @@ static hipError_t launch_fft_L(const Lines &ln, const void *in, void *out, const
     if (ln.elem_stride == 1 && ln.col_stride == ln.N && ln.nplanes * ln.ncols > 0 &&
         (ln.nplanes == 1 || ln.plane_stride == ln.ncols * ln.N)) {
         long nrows = ln.ncols * ln.nplanes;
-        auto kfn = k_dst_fft_rows<Ti, T, To, L, TPB>;
+        auto kfn = ln.in_tw ? k_dst_fft_rows<Ti, T, To, L, TPB, true> : k_dst_fft_rows<Ti, T, To, L, TPB, false>;
         if (shm > 65536) {
             hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
             if (e != hipSuccess) return e;
         }
         long grid = (nrows + cfg::LPB - 1) / cfg::LPB;
-        hipLaunchKernelGGL(kfn, dim3((unsigned)grid), dim3(cfg::THREADS), shm, st, pi, po, nrows, scale, ptw);
+        const RowMap rin{ln.N, ln.ny, ln.in_tw, ln.field}, rout{ln.N, ln.ny, ln.out_tw, ln.field};
+        hipLaunchKernelGGL(kfn, dim3((unsigned)grid), dim3(cfg::THREADS), shm, st, pi, po, nrows, scale, ptw, rin,
+                           rout);
         return hipGetLastError();
     }
     if (ln.col_stride != 1) return hipErrorInvalidValue;
@@ -413,13 +529,15 @@ static hipError_t launch_half_M(const Lines &ln, const void *in, void *out, cons
     const C2<T> *pa = (const C2<T> *)aux;
     if (ln.elem_stride == 1 && ln.col_stride == ln.N && (ln.nplanes == 1 || ln.plane_stride == ln.ncols * ln.N)) {
         long nrows = ln.ncols * ln.nplanes;
-        auto kfn = k_dst_half_rows<Ti, T, To, M, TPB>;
+        auto kfn = ln.in_tw ? k_dst_half_rows<Ti, T, To, M, TPB, true> : k_dst_half_rows<Ti, T, To, M, TPB, false>;
         if (shm > 65536) {
             hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
             if (e != hipSuccess) return e;
         }
         long grid = (nrows + cfg::LPB - 1) / cfg::LPB;
-        hipLaunchKernelGGL(kfn, dim3((unsigned)grid), dim3(cfg::THREADS), shm, st, pi, po, nrows, scale, pa);
+        const RowMap rin{ln.N, ln.ny, ln.in_tw, ln.field}, rout{ln.N, ln.ny, ln.out_tw, ln.field};
+        hipLaunchKernelGGL(kfn, dim3((unsigned)grid), dim3(cfg::THREADS), shm, st, pi, po, nrows, scale, pa, rin,
+                           rout);
         return hipGetLastError();
     }
     if (ln.col_stride != 1) return hipErrorInvalidValue;

@@ -26,6 +26,12 @@ struct Lines {
     long plane_stride;
     long col_stride;
     long elem_stride;
+    // Row passes only: input / output in the mode-tiled layout (tile width,
+    // a power of two; 0 = row-major), ny lines per right-hand side of `field`
+    // elements.  Tile t0 holds modes t0..t0+w-1 of all ny lines, element
+    // (j, k) at t0*ny + j*w + (k - t0), w = min(tw, N - t0).
+    int in_tw = 0, out_tw = 0, ny = 1;
+    long field = 0;
 };
 
 bool fft_supported(int N);  // N+1 a power of two, 2(N+1) <= HH_FFT_LMAX
@@ -48,6 +54,8 @@ struct TriHelm {
     long cstride;       // complex entries between operator tables
     const double *lamx, *lamy;   // eigenvalues 2cos(k pi/(N+1))
     int W, P, mmax;     // block shape (modes per block, chunks per mode, rows per chunk)
+    int nr = 1;         // right-hand sides per thread (sharing one operator): 1, 2 or 4
+    int tiled = 0;      // field in the mode-tiled layout of the row DSTs: tile width (a multiple of W), 0 = row-major
 };
 hipError_t launch_tri_helm(const TriHelm &t, DstTypes ty, const void *in, void *out, hipStream_t st);
 

@@ -133,7 +133,9 @@ int request_coef(const hh_request &r, std::vector<double> &out) {
 // solve's workspace (estimated; refined once the plan exists).
 size_t resident_bytes(const Shape &s, int cap, int max_batch) {
     const size_t tables = (size_t)max_batch * (s.dim == 2 ? 6 : 12) * s.sys() * 24 + 8 * (s.Nx + s.Ny);
-    const size_t ws = s.prec == HH_PREC_MIXED ? (size_t)cap * s.field() * 8 : 0;
+    // context scratch of a solve: the 2D mode-tiled spectrum (16 B per DOF in
+    // FP64, 8 B in FP32 plus the 8 B FP32 correction), the 3D FP32 correction
+    const size_t ws = (size_t)cap * s.field() * (s.dim == 2 ? 16 : (s.prec == HH_PREC_MIXED ? 8 : 0));
     return 2 * (size_t)cap * field_bytes(s) + tables + ws;
 }
 

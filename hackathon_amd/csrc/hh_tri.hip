@@ -79,18 +79,38 @@ template <typename T, int DIM> struct HelmBands {
 
 // Helmholtz problem: bands from HelmBands, right-hand side and solution in
 // the row-major field (element (b, m, q) at b*field + m + q*S).
+// the row-major field (element (b, m, q) at b*field + m + q*S), or, with
+// tw > 0, the mode-tiled one the row DSTs write for it (tiles of tw
+// neighbouring modes, tile t0 = m & ~(tw-1) holding its modes for all n rows
+// contiguously: b*field + t0*n + q*w + (m - t0), w = min(tw, nmodes - t0)),
+// where each thread's rows are whole w*16-byte segments one after another.
 template <typename T, typename Ti, typename To, int DIM> struct HelmProb {
     HelmBands<T, DIM> bands;
     const C2<Ti> *in;
     C2<To> *out;
-    long field, S;
-    struct Mode { typename HelmBands<T, DIM>::Mode bm; long m; };
+    long field, S, nmodes;
+    int tw;
+    struct Mode { typename HelmBands<T, DIM>::Mode bm; long off, step; };
     HH_DEV int n() const { return bands.n; }
-    HH_DEV Mode mode(long m, int bt) const { Mode md; md.bm = bands.mode(m, bt); md.m = m; return md; }
+    HH_DEV Mode mode(long m, int bt) const {
+        Mode md;
+        md.bm = bands.mode(m, bt);
+        if (tw == 0) {
+            md.off = m;
+            md.step = S;
+        } else {
+            const long t0 = m & ~(long)(tw - 1);
+            md.step = min((long)tw, nmodes - t0);
+            md.off = t0 * bands.n + (m - t0);
+        }
+        return md;
+    }
     HH_DEV Band<T> band(const Mode &md, int q) const { return bands.band(md.bm, q); }
-    HH_DEV C2<T> rhs(int b, const Mode &md, int q) const { return cconv<T, Ti>(in[b * field + md.m + (long)q * S]); }
+    HH_DEV C2<T> rhs(int b, const Mode &md, int q) const {
+        return cconv<T, Ti>(in[b * field + md.off + (long)q * md.step]);
+    }
     HH_DEV void store(int b, const Mode &md, int q, C2<T> x) const {
-        out[b * field + md.m + (long)q * S] = cconv<To, T>(x);
+        out[b * field + md.off + (long)q * md.step] = cconv<To, T>(x);
     }
 };
 
@@ -152,160 +172,255 @@ HH_DEV long xcd_tile(long b, long G) {
     return (b & 7) * (G8 >> 3) + (b >> 3);
 }
 
-template <typename T, int MMAX, int NT, class Prob>
-__global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P) {
+// NR right-hand sides that share the operator (indices o, o+nop, o+2 nop,
+// ... of a batch) ride through one thread: the operator-only quantities
+// (bands, pivots, the a~, c~ coefficients of the sweeps and the A, C of the
+// reduced rows) are computed once for all of them, only the right-hand-side
+// recurrences are repeated.  Block (mode tile, y): operator o = y % nop,
+// right-hand sides o + nop (NR (y / nop) + r), r < NR, those past the batch
+// are computed on a clamped index and not stored.
+template <typename T, int MMAX, int NR, int NT, class Prob>
+__global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P, int batch, int nop) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    // buffer b of the reduced rows: A at rA + b*RW, C at rC + b*RW, D at rD + b*RW
-    C2<T> *rA = reinterpret_cast<C2<T> *>(smem_raw);
-    C2<T> *rC = rA + 2 * P * W;
-    C2<T> *rD = rC + 2 * P * W;
+    // two buffers of the reduced rows, each [A | C | D_0 .. D_{NR-1}], rows x W
+    C2<T> *lds = reinterpret_cast<C2<T> *>(smem_raw);
+    const int RR = 2 * P;
+    const int RW = RR * W;
+    const int BUF = (2 + NR) * RW;
     const int n = prob.n();
     const int w = threadIdx.x % W, c = threadIdx.x / W;
     const long m = xcd_tile(blockIdx.x, gridDim.x) * W + w;
     const bool valid = (c < P) && (m < nmodes);
-    const int bt = blockIdx.y;
+    const int o = blockIdx.y % nop, jg = blockIdx.y / nop;
+    int bidx[NR];
+    bool bok[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int b = o + nop * (jg * NR + r);
+        bok[r] = b < batch;
+        bidx[r] = bok[r] ? b : (o < batch ? o : 0);
+    }
     // Idle lanes (c >= P, padding modes) run the same code on a clamped
     // chunk and mode so that every load is in bounds and no branch splits
     // the sweeps; their results are discarded.
     const int cq = c < P ? c : P - 1;
     const int s = (int)(((long)cq * n) / P), e = (int)(((long)(cq + 1) * n) / P) - 1;
     const int len = e - s + 1;   // 2 <= len <= MMAX
-    const auto md = prob.mode(m < nmodes ? m : nmodes - 1, bt);
+    const auto md = prob.mode(m < nmodes ? m : nmodes - 1, bidx[0]);
 
     // 0. issue every right-hand-side load of the chunk up front (rows past
     //    e repeat row e), so that the HBM latency is paid once per chunk and
     //    not once per row of the recurrence
-    C2<T> db[MMAX], ab[MMAX], cb[MMAX];
+    C2<T> db[NR][MMAX], ab[MMAX], cb[MMAX];
 #pragma unroll
-    for (int t = 0; t < MMAX; t++) db[t] = prob.rhs(bt, md, min(s + t, e));
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+        for (int t = 0; t < MMAX; t++) db[r][t] = prob.rhs(bidx[r], md, min(s + t, e));
     const Band<T> bs = prob.band(md, s);
-    const C2<T> sub_s = bs.sub, dia_s = bs.dia, sup_s = bs.sup, d_s = db[0];
+    const C2<T> sub_s = bs.sub, dia_s = bs.dia, sup_s = bs.sup;
+    C2<T> d_s[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) d_s[r] = db[r][0];
 
     // 1. downward sweep over rows s+1..e: row s+t becomes
     //    ab[t] x_s + x_{s+t} + cb[t] x_{s+t+1} = db[t]
-    C2<T> pa = mk<T>(0, 0), pc = mk<T>(0, 0), pd = mk<T>(0, 0);
+    C2<T> pa = mk<T>(0, 0), pc = mk<T>(0, 0), pd[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) pd[r] = mk<T>(0, 0);
 #pragma unroll
     for (int t = 1; t < MMAX; t++) {
         const Band<T> bq = prob.band(md, min(s + t, e));
-        const C2<T> a = bq.sub, d = db[t];
+        const C2<T> a = bq.sub;
         const C2<T> den = (t == 1) ? bq.dia : cfms<T>(a, pc, bq.dia);
-        const C2<T> r = crcp<T>(den);
-        const C2<T> na = (t == 1) ? cmul<T>(a, r) : cneg<T>(cmul<T>(cmul<T>(a, pa), r));
-        const C2<T> nd = (t == 1) ? cmul<T>(d, r) : cmul<T>(cfms<T>(a, pd, d), r);
-        const C2<T> nc = cmul<T>(bq.sup, r);
+        const C2<T> rc = crcp<T>(den);
+        const C2<T> na = (t == 1) ? cmul<T>(a, rc) : cneg<T>(cmul<T>(cmul<T>(a, pa), rc));
+        const C2<T> nc = cmul<T>(bq.sup, rc);
+        C2<T> nd[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) nd[r] = (t == 1) ? cmul<T>(db[r][t], rc) : cmul<T>(cfms<T>(a, pd[r], db[r][t]), rc);
         if (t < len) {   // selects, not a branch
-            ab[t] = na; cb[t] = nc; db[t] = nd;
-            pa = na; pc = nc; pd = nd;
+            ab[t] = na; cb[t] = nc;
+            pa = na; pc = nc;
+#pragma unroll
+            for (int r = 0; r < NR; r++) { db[r][t] = nd[r]; pd[r] = nd[r]; }
         }
     }
     // reduced row 2c+1 (x_e): pa x_s + x_e + pc x_{e+1} = pd  (already normalised)
-    C2<T> A1 = pa, C1 = pc, D1 = pd;
+    C2<T> A1 = pa, C1 = pc, D1[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) D1[r] = pd[r];
     // 2. upward sweep over rows e-2..s+1 (row e-1 keeps its downward form):
     //    row s+t becomes ab[t] x_s + x_{s+t} + cb[t] x_e = db[t]
 #pragma unroll
     for (int t = MMAX - 3; t >= 1; t--) {
         const C2<T> cbt = cb[t];
         const C2<T> na = cfms<T>(cbt, ab[t + 1], ab[t]);
-        const C2<T> nd = cfms<T>(cbt, db[t + 1], db[t]);
         const C2<T> nc = cneg<T>(cmul<T>(cbt, cb[t + 1]));
-        if (t <= len - 3) { ab[t] = na; db[t] = nd; cb[t] = nc; }
+        C2<T> nd[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) nd[r] = cfms<T>(cbt, db[r][t + 1], db[r][t]);
+        if (t <= len - 3) {
+            ab[t] = na; cb[t] = nc;
+#pragma unroll
+            for (int r = 0; r < NR; r++) db[r][t] = nd[r];
+        }
     }
     // reduced row 2c (x_s): sub_s x_{s-1} + B x_s + C x_e = D
-    C2<T> B0, C0, D0;
+    C2<T> B0, C0, D0[NR];
     if (len >= 3) {
         B0 = cfms<T>(sup_s, ab[1], dia_s);
         C0 = cneg<T>(cmul<T>(sup_s, cb[1]));
-        D0 = cfms<T>(sup_s, db[1], d_s);
+#pragma unroll
+        for (int r = 0; r < NR; r++) D0[r] = cfms<T>(sup_s, db[r][1], d_s[r]);
     } else {
-        B0 = dia_s; C0 = sup_s; D0 = d_s;
+        B0 = dia_s; C0 = sup_s;
+#pragma unroll
+        for (int r = 0; r < NR; r++) D0[r] = d_s[r];
     }
+    C2<T> A0;
     {
-        C2<T> r = crcp<T>(B0);
-        B0 = cmul<T>(sub_s, r);  // reuse B0 as the normalised sub coefficient A0
-        C0 = cmul<T>(C0, r);
-        D0 = cmul<T>(D0, r);
+        const C2<T> rc = crcp<T>(B0);
+        A0 = cmul<T>(sub_s, rc);
+        C0 = cmul<T>(C0, rc);
+#pragma unroll
+        for (int r = 0; r < NR; r++) D0[r] = cmul<T>(D0[r], rc);
     }
-    C2<T> A0 = B0;
     if (!valid) {  // identity rows for idle threads and padding modes
         A0 = C0 = A1 = C1 = mk<T>(0, 0);
-        D0 = D1 = mk<T>(0, 0);
+#pragma unroll
+        for (int r = 0; r < NR; r++) D0[r] = D1[r] = mk<T>(0, 0);
     }
     // 3. PCR on the reduced system, rows r0 = 2c, r1 = 2c+1, layout [row][W].
     //    Two LDS buffers: step i reads buffer i%2 and writes the other, so
     //    one barrier per step suffices.
-    const int R = 2 * P;
-    const int r0 = 2 * c, r1 = 2 * c + 1;
-    const int RW = 3 * R * W;   // complex entries per buffer (A, C, D)
+    // state of the thread's two reduced rows q = 2c + h, h = 0 (x_s), 1 (x_e)
+    C2<T> PA[2] = {A0, A1}, PC[2] = {C0, C1}, PD[2][NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) { PD[0][r] = D0[r]; PD[1][r] = D1[r]; }
     if (c < P) {
-        rA[r0 * W + w] = A0; rC[r0 * W + w] = C0; rD[r0 * W + w] = D0;
-        rA[r1 * W + w] = A1; rC[r1 * W + w] = C1; rD[r1 * W + w] = D1;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int q = 2 * c + h;
+            lds[q * W + w] = PA[h];
+            lds[RW + q * W + w] = PC[h];
+#pragma unroll
+            for (int r = 0; r < NR; r++) lds[(2 + r) * RW + q * W + w] = PD[h][r];
+        }
     }
     int cur = 0;
-    for (int dd = 1; dd < R; dd <<= 1) {
+    for (int dd = 1; dd < RR; dd <<= 1) {
         __syncthreads();
-        const C2<T> *sA = rA + cur * RW, *sC = rC + cur * RW, *sD = rD + cur * RW;
-        C2<T> *tA = rA + (cur ^ 1) * RW, *tC = rC + (cur ^ 1) * RW, *tD = rD + (cur ^ 1) * RW;
+        const C2<T> *sb = lds + cur * BUF;
+        C2<T> *tb = lds + (cur ^ 1) * BUF;
         if (c < P) {
-            // row r: A x_{r-d} + x_r + C x_{r+d} = D, eliminated against rows r-d and r+d
-            auto step = [&](int r, C2<T> &A, C2<T> &C, C2<T> &D) {
+            // row q: A x_{q-d} + x_q + C x_{q+d} = D, eliminated against rows q-d and q+d
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int q = 2 * c + h;
+                // neighbours outside the system read the row itself and are zeroed
                 const C2<T> z = mk<T>(0, 0);
-                C2<T> aL = z, cL = z, dL = z, aU = z, cU = z, dU = z;
-                int q = r - dd;
-                if (q >= 0) { aL = sA[q * W + w]; cL = sC[q * W + w]; dL = sD[q * W + w]; }
-                q = r + dd;
-                if (q < R) { aU = sA[q * W + w]; cU = sC[q * W + w]; dU = sD[q * W + w]; }
-                const C2<T> b = csub<T>(mk<T>(1, 0), cadd<T>(cmul<T>(A, cL), cmul<T>(C, aU)));
-                const C2<T> na = cneg<T>(cmul<T>(A, aL)), nc = cneg<T>(cmul<T>(C, cU));
-                const C2<T> nd = csub<T>(D, cadd<T>(cmul<T>(A, dL), cmul<T>(C, dU)));
-                const C2<T> ib = crcp<T>(b);
-                A = cmul<T>(na, ib); C = cmul<T>(nc, ib); D = cmul<T>(nd, ib);
-                tA[r * W + w] = A; tC[r * W + w] = C; tD[r * W + w] = D;
-            };
-            step(r0, A0, C0, D0);
-            step(r1, A1, C1, D1);
+                const int qL = q - dd, qU = q + dd;
+                const bool hL = qL >= 0, hU = qU < RR;
+                const int iL = (hL ? qL : q) * W + w, iU = (hU ? qU : q) * W + w;
+                C2<T> aL = sb[iL], cL = sb[RW + iL], aU = sb[iU], cU = sb[RW + iU], dL[NR], dU[NR];
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    dL[r] = sb[(2 + r) * RW + iL];
+                    dU[r] = sb[(2 + r) * RW + iU];
+                }
+                if (!hL) {
+                    aL = z; cL = z;
+#pragma unroll
+                    for (int r = 0; r < NR; r++) dL[r] = z;
+                }
+                if (!hU) {
+                    aU = z; cU = z;
+#pragma unroll
+                    for (int r = 0; r < NR; r++) dU[r] = z;
+                }
+                const C2<T> A = PA[h], C = PC[h];
+                const C2<T> bb = csub<T>(mk<T>(1, 0), cadd<T>(cmul<T>(A, cL), cmul<T>(C, aU)));
+                const C2<T> ib = crcp<T>(bb);
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                    const C2<T> nd = csub<T>(PD[h][r], cadd<T>(cmul<T>(A, dL[r]), cmul<T>(C, dU[r])));
+                    PD[h][r] = cmul<T>(nd, ib);
+                    tb[(2 + r) * RW + q * W + w] = PD[h][r];
+                }
+                PA[h] = cmul<T>(cneg<T>(cmul<T>(A, aL)), ib);
+                PC[h] = cmul<T>(cneg<T>(cmul<T>(C, cU)), ib);
+                tb[q * W + w] = PA[h];
+                tb[RW + q * W + w] = PC[h];
+            }
         }
         cur ^= 1;
     }
-    if (!valid) return;
-    const C2<T> xs = D0, xe = D1;
-    // 4. back-substitution of the interior and store
-    prob.store(bt, md, s, xs);
 #pragma unroll
-    for (int t = 1; t < MMAX - 1; t++) {
-        if (t < len - 1) {
-            C2<T> x = csub<T>(db[t], cadd<T>(cmul<T>(ab[t], xs), cmul<T>(cb[t], xe)));
-            prob.store(bt, md, s + t, x);
+    for (int r = 0; r < NR; r++) { D0[r] = PD[0][r]; D1[r] = PD[1][r]; }
+    if (!valid) return;
+    // 4. back-substitution of the interior and store
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        if (!bok[r]) continue;
+        const C2<T> xs = D0[r], xe = D1[r];
+        const int b = bidx[r];
+        prob.store(b, md, s, xs);
+#pragma unroll
+        for (int t = 1; t < MMAX - 1; t++) {
+            if (t < len - 1) {
+                C2<T> x = csub<T>(db[r][t], cadd<T>(cmul<T>(ab[t], xs), cmul<T>(cb[t], xe)));
+                prob.store(b, md, s + t, x);
+            }
         }
+        prob.store(b, md, e, xe);
     }
-    prob.store(bt, md, e, xe);
 }
 
-template <typename T, int MM, class Prob>
-static hipError_t launch_tri_prob(const Prob &pb, int n, long nmodes, int batch, int W, int P, hipStream_t st) {
-    if (n < 2) return hipErrorInvalidValue;
+// LDS bytes of the reduced system: two buffers of A, C and NR right-hand sides.
+inline size_t tri_lds_bytes(size_t elem, int P, int W, int nr) { return elem * 2 * (2 + nr) * 2 * (size_t)P * W; }
+
+template <typename T, int MM, int NR, class Prob>
+static hipError_t launch_tri_nr(const Prob &pb, int n, long nmodes, int batch, int nop, int W, int P,
+                                hipStream_t st) {
+    if (n < 2 || batch < 1 || nop < 1) return hipErrorInvalidValue;
     // every chunk needs at least two rows, at most MM rows
     if (P < 1 || P > n / 2 || (n + P - 1) / P > MM || W * P > 1024) return hipErrorInvalidValue;
     int threads = ((W * P + 63) / 64) * 64;
-    size_t shm = sizeof(C2<T>) * 2 * 3 * 2 * (size_t)P * W;   // two buffers of A, C, D
+    const size_t shm = tri_lds_bytes(sizeof(C2<T>), P, W, NR);
+    if (shm > 160 * 1024) return hipErrorInvalidValue;
     // 512-thread bound: at most 256 VGPRs per lane and two waves per SIMD.
     // (16 FP64 rows spill a few registers there; measured faster than the
     // spill-free 256-thread bound, which leaves one wave per SIMD.)
-    auto kfn = threads <= 512 ? k_tri<T, MM, 512, Prob> : k_tri<T, MM, 1024, Prob>;
+    auto kfn = threads <= 512 ? k_tri<T, MM, NR, 512, Prob> : k_tri<T, MM, NR, 1024, Prob>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
     }
-    dim3 grid((unsigned)((nmodes + W - 1) / W), (unsigned)batch);
-    hipLaunchKernelGGL(kfn, grid, dim3(threads), shm, st, pb, nmodes, W, P);
+    const int per_op = (batch + nop - 1) / nop;
+    const int groups = (per_op + NR - 1) / NR;
+    dim3 grid((unsigned)((nmodes + W - 1) / W), (unsigned)(nop * groups));
+    hipLaunchKernelGGL(kfn, grid, dim3(threads), shm, st, pb, nmodes, W, P, batch, nop);
     return hipGetLastError();
 }
 
+template <typename T, int MM, class Prob>
+static hipError_t launch_tri_prob(const Prob &pb, int n, long nmodes, int batch, int nop, int W, int P, int nr,
+                                  hipStream_t st) {
+    if (nr == 1) return launch_tri_nr<T, MM, 1>(pb, n, nmodes, batch, nop, W, P, st);
+    if constexpr (sizeof(T) == 8) {   // several right-hand sides per thread: FP64 elimination only
+        if (nr == 2) return launch_tri_nr<T, MM, 2>(pb, n, nmodes, batch, nop, W, P, st);
+        if (nr == 4) return launch_tri_nr<T, MM, 4>(pb, n, nmodes, batch, nop, W, P, st);
+    }
+    return hipErrorInvalidValue;
+}
+
 void tri_shape(int n, long nmodes, int prec, int *W, int *P, int *mmax) {
-    // Measured on MI355X (FP64, k = 50 compact operators): 256-thread blocks
-    // win at every length tried; 16 rows per thread above n = 512 (fewer
-    // chunks, a shallower reduced system), 8 below.  n = 1023, 64 x 1023
-    // modes: W = 4 x 16 rows 894 us against 1297 us for W = 4 x 8 rows;
-    // n = 255, 255^2 modes: W = 8 x 8 rows 209 us against 286 us for 16 x 8.
+    // Measured on MI355X (FP64, k = 50 compact operators, row-major field):
+    // 256-thread blocks win at every length tried; 16 rows per thread above
+    // n = 512 (fewer chunks, a shallower reduced system), 8 below.  n = 1023,
+    // 64 x 1023 modes: W = 4 x 16 rows 809 us against 899 us for W = 2 x 8
+    // rows and 917 us for W = 8 x 16; n = 255, 255^2 modes: W = 8 x 8 rows
+    // 207 us against 283 us for 16 x 8.
     int mm = n > 512 ? 16 : 8;
     if (n <= 16) mm = 4;
     (void)prec;
@@ -337,7 +452,11 @@ static hipError_t launch_tri_d(const TriHelm &t, const void *in, void *out, hipS
     long nmodes;
     if (t.dim == 2) { pb.bands.n = t.Ny; nmodes = t.Nx; pb.S = t.Nx; }
     else { pb.bands.n = t.Nz; nmodes = (long)t.Nx * t.Ny; pb.S = (long)t.Nx * t.Ny; }
-    return launch_tri_prob<T, MM>(pb, pb.bands.n, nmodes, t.batch, t.W, t.P, st);
+    pb.nmodes = nmodes;
+    // tiles are powers of two holding whole blocks of W modes
+    pb.tw = t.tiled;
+    if (t.tiled && ((t.tiled & (t.tiled - 1)) || t.tiled % t.W)) return hipErrorInvalidValue;
+    return launch_tri_prob<T, MM>(pb, pb.bands.n, nmodes, t.batch, pb.bands.nop, t.W, t.P, t.nr, st);
 }
 
 template <typename T, typename Ti, typename To, int MM>
@@ -374,9 +493,9 @@ hipError_t launch_compact(const Compact &cd, hipStream_t st) {
     int W, P, mm;
     tri_shape(cd.n, cd.nmodes, F64, &W, &P, &mm);
     switch (mm) {
-    case 4: return launch_tri_prob<double, 4>(pb, cd.n, cd.nmodes, cd.batch, W, P, st);
-    case 8: return launch_tri_prob<double, 8>(pb, cd.n, cd.nmodes, cd.batch, W, P, st);
-    case 16: return launch_tri_prob<double, 16>(pb, cd.n, cd.nmodes, cd.batch, W, P, st);
+    case 4: return launch_tri_nr<double, 4, 1>(pb, cd.n, cd.nmodes, cd.batch, 1, W, P, st);
+    case 8: return launch_tri_nr<double, 8, 1>(pb, cd.n, cd.nmodes, cd.batch, 1, W, P, st);
+    case 16: return launch_tri_nr<double, 16, 1>(pb, cd.n, cd.nmodes, cd.batch, 1, W, P, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -136,12 +136,21 @@ int hh_plan_last_refine(const hh_plan *plan, int *sweeps, double *rel_residual);
  * queued on the context's stream, then queues the table copies on it. */
 int hh_plan_set_operators(hh_plan *plan, int nop, const double *coef);
 /* Block shape of the tridiagonal kernel: W neighbouring modes per block,
- * rows_per_thread in {4, 8, 16} (the chunk count follows).  Set by the
- * autotuner; (0, 0) restores the built-in heuristic.  Shapes differ in
- * speed and in roundoff only; for a given shape every right-hand side gives
- * the same bits whatever else is in its batch. */
-int hh_plan_set_tri_shape(hh_plan *plan, int W, int rows_per_thread);
-int hh_plan_get_tri_shape(const hh_plan *plan, int *W, int *P, int *rows_per_thread);
+ * rows_per_thread in {4, 8, 16} (the chunk count follows), rhs_per_thread
+ * in {1, 2, 4} right-hand sides of one operator handled together by a
+ * thread (the operator's pivots are computed once for them).  Set by the
+ * autotuner; (0, 0, *) restores the built-in heuristic.  Shapes differ in
+ * speed only: every right-hand side gives the same bits for a given W and
+ * rows_per_thread, whatever else is in its batch and whatever rhs_per_thread. */
+int hh_plan_set_tri_shape(hh_plan *plan, int W, int rows_per_thread, int rhs_per_thread);
+int hh_plan_get_tri_shape(const hh_plan *plan, int *W, int *P, int *rows_per_thread, int *rhs_per_thread);
+/* 2D FFT path: layout of the spectrum between the row DSTs and the mode
+ * solves.  0 = row-major, in place in d_sol; a power of two tw (a multiple
+ * of the tridiagonal W) = tiles of tw neighbouring modes holding all rows
+ * contiguously, in workspace (one more field per right-hand side); -1 =
+ * tiles of the tridiagonal W.  Default 0.  Tiles make the mode solves
+ * faster and the row DSTs slower; layouts differ in speed only. */
+int hh_plan_set_tile_width(hh_plan *plan, int tw);
 
 /* ---- graphs: a fixed list of solves replayed with one launch ----
  * No reference counterpart (the reference solves one problem per call): for

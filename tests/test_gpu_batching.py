@@ -112,3 +112,55 @@ def test_set_operators_matches_new_plan(H, ctx, oracle):
     assert relerr(swapped[0].ravel(), probs[3]["sol"]) <= 1e-12
     with pytest.raises(H.HHError):
         plan.set_operators(ops + ops)        # more operators than slots
+
+
+@pytest.mark.parametrize("nop,batch", [(1, 7), (2, 7), (3, 8)])
+def test_rhs_per_thread_same_bits(H, ctx, oracle, nop, batch):
+    """Several right-hand sides per thread (sharing an operator) give the
+    same bits as one per thread, for batches that do not fill the groups."""
+    import torch
+    N = 255
+    probs = [oracle.solve2d(o, N, 1, [k, 0.0, 3, 2, 0.3], threads=8) for o, k in [(4, 12.0), (6, 40.0), (6, 75.0)]]
+    ops = [p["coef"] for p in probs[:nop]]
+    plan = H.Plan2D(ctx, N, N, ops)
+    g = torch.Generator(device="cuda").manual_seed(nop * 10 + batch)
+    rhs = torch.randn((batch, N, N), dtype=torch.complex128, device="cuda", generator=g)
+    ref = None
+    for W, rows in ((2, 8), (4, 4)):
+        for r in (1, 2, 4):
+            H._check(H.lib().hh_plan_set_tri_shape(plan.h, W, rows, r))
+            out = plan.solve(rhs).cpu().numpy()
+            say(f"nop={nop} batch={batch} W={W} rows={rows} rhs/thread={r}")
+            if r == 1:
+                ref = out
+            else:
+                assert np.array_equal(out, ref)
+    # and the solution of right-hand side b matches the oracle for its operator b % nop
+    rhs_o = dev(np.stack([probs[b % nop]["rhs"] for b in range(batch)]))
+    H._check(H.lib().hh_plan_set_tri_shape(plan.h, 2, 8, 4))
+    sol = plan.solve(rhs_o).cpu().numpy()
+    for b in range(batch):
+        assert relerr(sol[b].ravel(), probs[b % nop]["sol"]) <= 1e-12
+
+
+def test_spectrum_layouts_same_bits(H, ctx, oracle):
+    """Row-major and mode-tiled spectra (any tile width) give the same bits."""
+    import torch
+    N = 127
+    s = oracle.solve2d(6, N, 1, [33.0, 0.0, 5, 3, 0.4], threads=8)
+    plan = H.Plan2D(ctx, N, N, s["coef"])
+    H._check(H.lib().hh_plan_set_tri_shape(plan.h, 4, 8, 1))   # tiles must hold whole blocks of 4 modes
+    g = torch.Generator(device="cuda").manual_seed(5)
+    rhs = torch.randn((3, N, N), dtype=torch.complex128, device="cuda", generator=g)
+    rhs[0] = dev(s["rhs"]).reshape(N, N)
+    outs = []
+    for tw in (0, -1, 8, 16, 64):
+        H._check(H.lib().hh_plan_set_tile_width(plan.h, tw))
+        outs.append(plan.solve(rhs).cpu().numpy())
+        say(f"tile width {tw}")
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    assert relerr(outs[0][0].ravel(), s["sol"]) <= 1e-12
+    with pytest.raises(H.HHError):   # a tile narrower than the block is refused at launch
+        H._check(H.lib().hh_plan_set_tile_width(plan.h, 2))
+        plan.solve(rhs)
