@@ -281,76 +281,127 @@ hipError_t launch_rhs3d(const Rhs3D &r, void *F, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------- residual
-template <typename To>
-__global__ void k_residual2d(int Nx, int Ny, long total, const z2 *coef, int nop, const z2 *u, const z2 *F,
-                             C2<To> *res) {
-    long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= total) return;
-    const long Nxy = (long)Nx * Ny;
-    const long b = id / Nxy, p = id % Nxy;
-    const int i = (int)(p % Nx), j = (int)(p / Nx);
-    const z2 *ub = u + b * Nxy;
-    const z2 *cq = coef + 6 * ((long)(b % nop) * Ny + j);  // bm, am, b, a, bp, ap
-    auto U = [&](int ii, int jj) -> z2 {
-        return (ii >= 0 && ii < Nx && jj >= 0 && jj < Ny) ? ub[ii + (long)jj * Nx] : zr(0.0);
-    };
-    z2 top = za(za(zm(cq[4], U(i - 1, j + 1)), zm(cq[5], U(i, j + 1))), zm(cq[4], U(i + 1, j + 1)));
-    z2 mid = za(za(zm(cq[2], U(i - 1, j)), zm(cq[3], U(i, j))), zm(cq[2], U(i + 1, j)));
-    z2 bot = za(za(zm(cq[0], U(i - 1, j - 1)), zm(cq[1], U(i, j - 1))), zm(cq[0], U(i + 1, j - 1)));
-    res[id] = cconv<To, double>(zd(za(za(top, mid), bot), F[id]));
-}
+// res = A u - F with the ghosts of u zero (2D_Program/main_functions/
+// residual.c:3-34, 3D_Program/main_functions/residual.c:42-96).  Each thread
+// marches along the last axis over a chunk of rows (2D) or layers (3D),
+// keeping the 3 x 3 (x 3) neighbourhood in registers so that every value of
+// u is loaded about once per thread row; the coefficient row of a step is
+// the same for the whole block.  The sums keep the order of the reference.
+constexpr int RES_ROWS = 8;   // rows (2D) or layers (3D) per thread
 
 template <typename To>
-__global__ void k_residual3d(int Nx, int Ny, int Nz, long total, const z2 *coef, int nop, const z2 *u, const z2 *F,
-                             C2<To> *res) {
-    long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= total) return;
-    const long Nxy = (long)Nx * Ny, N3 = Nxy * Nz;
-    const long b = id / N3, p = id % N3;
-    const int i = (int)(p % Nx), j = (int)((p / Nx) % Ny), l = (int)(p / Nxy);
-    const z2 *ub = u + b * N3;
-    auto U = [&](int ii, int jj, int ll) -> z2 {
-        return (ii >= 0 && ii < Nx && jj >= 0 && jj < Ny && ll >= 0 && ll < Nz) ? ub[ii + (long)jj * Nx + (long)ll * Nxy]
-                                                                               : zr(0.0);
+__global__ __launch_bounds__(256) void k_residual2d(int Nx, int Ny, int nchunk, const z2 *coef, int nop, const z2 *u,
+                                                    const z2 *F, C2<To> *res, int b0) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int j0 = blockIdx.y * RES_ROWS;
+    const int b = b0 + (int)blockIdx.z;
+    if (i >= Nx) return;
+    (void)nchunk;
+    const long Nxy = (long)Nx * Ny;
+    const z2 *ub = u + b * Nxy;
+    const z2 *cb = coef + 6L * (b % nop) * Ny;   // per row: bm, am, b, a, bp, ap
+    auto row = [&](int jj, z2 &l, z2 &c, z2 &r) {
+        if (jj < 0 || jj >= Ny) { l = c = r = zr(0.0); return; }
+        const z2 *p = ub + (long)jj * Nx;
+        c = p[i];
+        l = i > 0 ? p[i - 1] : zr(0.0);
+        r = i + 1 < Nx ? p[i + 1] : zr(0.0);
     };
-    z2 acc = zr(0.0);
-    // levels l-1 (am..dm), l (a..d), l+1 (ap..dp)
-    const int off[3] = {8, 0, 4};
-    for (int q = 0; q < 3; q++) {
-        const z2 *c = coef + 12 * ((long)(b % nop) * Nz + l) + off[q];
-        int ll = l + q - 1;
-        z2 corner = za(za(za(U(i + 1, j + 1, ll), U(i - 1, j + 1, ll)), U(i + 1, j - 1, ll)), U(i - 1, j - 1, ll));
-        z2 xn = za(U(i + 1, j, ll), U(i - 1, j, ll));
-        z2 yn = za(U(i, j + 1, ll), U(i, j - 1, ll));
-        z2 v = za(za(za(zm(c[3], U(i, j, ll)), zm(c[0], corner)), zm(c[1], xn)), zm(c[2], yn));
-        acc = za(acc, v);
+    z2 lm, cm, rm, l0, c0, r0, lp, cp, rp;
+    row(j0 - 1, lm, cm, rm);
+    row(j0, l0, c0, r0);
+    const int j1 = min(j0 + RES_ROWS, Ny);
+    for (int j = j0; j < j1; j++) {
+        row(j + 1, lp, cp, rp);
+        const z2 *cq = cb + 6L * j;
+        const z2 top = za(za(zm(cq[4], lp), zm(cq[5], cp)), zm(cq[4], rp));
+        const z2 mid = za(za(zm(cq[2], l0), zm(cq[3], c0)), zm(cq[2], r0));
+        const z2 bot = za(za(zm(cq[0], lm), zm(cq[1], cm)), zm(cq[0], rm));
+        const long id = b * Nxy + (long)j * Nx + i;
+        res[id] = cconv<To, double>(zd(za(za(top, mid), bot), F[id]));
+        lm = l0; cm = c0; rm = r0;
+        l0 = lp; c0 = cp; r0 = rp;
     }
-    res[id] = cconv<To, double>(zd(acc, F[id]));
+}
+
+// 3 x 3 neighbourhood of (i, j) in layer ll of one right-hand side
+struct Nb3 { z2 v[3][3]; };   // v[dj+1][di+1]
+
+template <typename To>
+__global__ __launch_bounds__(256) void k_residual3d(int Nx, int Ny, int Nz, int nchunk, const z2 *coef, int nop,
+                                                    const z2 *u, const z2 *F, C2<To> *res, int b0) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int zc = (int)blockIdx.z % nchunk;
+    const int b = b0 + (int)blockIdx.z / nchunk;
+    if (i >= Nx || j >= Ny) return;
+    const long Nxy = (long)Nx * Ny, N3 = Nxy * Nz;
+    const z2 *ub = u + b * N3;
+    auto plane = [&](int ll, Nb3 &p) {
+#pragma unroll
+        for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+            for (int di = -1; di <= 1; di++) {
+                const int ii = i + di, jj = j + dj;
+                const bool in = ll >= 0 && ll < Nz && ii >= 0 && ii < Nx && jj >= 0 && jj < Ny;
+                p.v[dj + 1][di + 1] = in ? ub[ii + (long)jj * Nx + (long)ll * Nxy] : zr(0.0);
+            }
+    };
+    // one layer's contribution, coefficients c = (a, b, c, d) of that layer set
+    auto layer = [&](const Nb3 &p, const z2 *c) -> z2 {
+        const z2 corner = za(za(za(p.v[2][2], p.v[2][0]), p.v[0][2]), p.v[0][0]);
+        const z2 xn = za(p.v[1][2], p.v[1][0]);
+        const z2 yn = za(p.v[2][1], p.v[0][1]);
+        return za(za(za(zm(c[3], p.v[1][1]), zm(c[0], corner)), zm(c[1], xn)), zm(c[2], yn));
+    };
+    const int l0 = zc * RES_ROWS, l1 = min(l0 + RES_ROWS, Nz);
+    Nb3 pm, pc, pp;
+    plane(l0 - 1, pm);
+    plane(l0, pc);
+    for (int l = l0; l < l1; l++) {
+        plane(l + 1, pp);
+        const z2 *cl = coef + 12 * ((long)(b % nop) * Nz + l);   // a..d, ap..dp, am..dm
+        z2 acc = layer(pm, cl + 8);
+        acc = za(acc, layer(pc, cl));
+        acc = za(acc, layer(pp, cl + 4));
+        const long id = b * N3 + (long)l * Nxy + (long)j * Nx + i;
+        res[id] = cconv<To, double>(zd(acc, F[id]));
+        pm = pc;
+        pc = pp;
+    }
 }
 
 hipError_t launch_residual2d(int Nx, int Ny, int batch, const void *coef, int nop, const void *u, const void *F,
                              void *res, int res_f32, hipStream_t st) {
-    long total = (long)Nx * Ny * batch;
-    dim3 g((unsigned)((total + 255) / 256));
-    if (res_f32)
-        hipLaunchKernelGGL(k_residual2d<float>, g, dim3(256), 0, st, Nx, Ny, total, (const z2 *)coef, nop,
-                           (const z2 *)u, (const z2 *)F, (float2 *)res);
-    else
-        hipLaunchKernelGGL(k_residual2d<double>, g, dim3(256), 0, st, Nx, Ny, total, (const z2 *)coef, nop,
-                           (const z2 *)u, (const z2 *)F, (z2 *)res);
+    const int nchunk = (Ny + RES_ROWS - 1) / RES_ROWS;
+    for (int b0 = 0; b0 < batch; b0 += 65535) {   // grid z holds at most 65535 right-hand sides
+        const int nb = batch - b0 < 65535 ? batch - b0 : 65535;
+        dim3 g((unsigned)((Nx + 255) / 256), (unsigned)nchunk, (unsigned)nb);
+        if (res_f32)
+            hipLaunchKernelGGL(k_residual2d<float>, g, dim3(256), 0, st, Nx, Ny, nchunk, (const z2 *)coef, nop,
+                               (const z2 *)u, (const z2 *)F, (float2 *)res, b0);
+        else
+            hipLaunchKernelGGL(k_residual2d<double>, g, dim3(256), 0, st, Nx, Ny, nchunk, (const z2 *)coef, nop,
+                               (const z2 *)u, (const z2 *)F, (z2 *)res, b0);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_residual3d(int Nx, int Ny, int Nz, int batch, const void *coef, int nop, const void *u,
                              const void *F, void *res, int res_f32, hipStream_t st) {
-    long total = (long)Nx * Ny * Nz * batch;
-    dim3 g((unsigned)((total + 255) / 256));
-    if (res_f32)
-        hipLaunchKernelGGL(k_residual3d<float>, g, dim3(256), 0, st, Nx, Ny, Nz, total, (const z2 *)coef, nop,
-                           (const z2 *)u, (const z2 *)F, (float2 *)res);
-    else
-        hipLaunchKernelGGL(k_residual3d<double>, g, dim3(256), 0, st, Nx, Ny, Nz, total, (const z2 *)coef, nop,
-                           (const z2 *)u, (const z2 *)F, (z2 *)res);
+    const int nchunk = (Nz + RES_ROWS - 1) / RES_ROWS;
+    const int per = 65535 / nchunk;   // right-hand sides per launch (grid z)
+    if (per < 1) return hipErrorInvalidValue;
+    for (int b0 = 0; b0 < batch; b0 += per) {
+        const int nb = batch - b0 < per ? batch - b0 : per;
+        dim3 g((unsigned)((Nx + 63) / 64), (unsigned)((Ny + 3) / 4), (unsigned)(nchunk * nb));
+        if (res_f32)
+            hipLaunchKernelGGL(k_residual3d<float>, g, dim3(64, 4), 0, st, Nx, Ny, Nz, nchunk, (const z2 *)coef, nop,
+                               (const z2 *)u, (const z2 *)F, (float2 *)res, b0);
+        else
+            hipLaunchKernelGGL(k_residual3d<double>, g, dim3(64, 4), 0, st, Nx, Ny, Nz, nchunk, (const z2 *)coef,
+                               nop, (const z2 *)u, (const z2 *)F, (z2 *)res, b0);
+    }
     return hipGetLastError();
 }
 
