@@ -293,7 +293,11 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
     // 3. PCR on the reduced system, rows r0 = 2c, r1 = 2c+1, layout [row][W].
     //    Two LDS buffers: step i reads buffer i%2 and writes the other, so
     //    one barrier per step suffices.
-    // state of the thread's two reduced rows q = 2c + h, h = 0 (x_s), 1 (x_e)
+    // state of the thread's two reduced rows q = 2c + h, h = 0 (x_s), 1 (x_e).
+    // In LDS row q sits at slot (q & 1) P + (q >> 1): the rows one step of a
+    // wave touches are then contiguous (a plain [q][W] layout puts chunk c
+    // and c+2 on the same banks)
+    auto slot = [&](int q) { return ((q & 1) * P + (q >> 1)) * W + w; };
     C2<T> PA[2] = {A0, A1}, PC[2] = {C0, C1}, PD[2][NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) { PD[0][r] = D0[r]; PD[1][r] = D1[r]; }
@@ -301,10 +305,10 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int q = 2 * c + h;
-            lds[q * W + w] = PA[h];
-            lds[RW + q * W + w] = PC[h];
+            lds[slot(q)] = PA[h];
+            lds[RW + slot(q)] = PC[h];
 #pragma unroll
-            for (int r = 0; r < NR; r++) lds[(2 + r) * RW + q * W + w] = PD[h][r];
+            for (int r = 0; r < NR; r++) lds[(2 + r) * RW + slot(q)] = PD[h][r];
         }
     }
     int cur = 0;
@@ -321,7 +325,7 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
                 const C2<T> z = mk<T>(0, 0);
                 const int qL = q - dd, qU = q + dd;
                 const bool hL = qL >= 0, hU = qU < RR;
-                const int iL = (hL ? qL : q) * W + w, iU = (hU ? qU : q) * W + w;
+                const int iL = slot(hL ? qL : q), iU = slot(hU ? qU : q);
                 C2<T> aL = sb[iL], cL = sb[RW + iL], aU = sb[iU], cU = sb[RW + iU], dL[NR], dU[NR];
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
@@ -345,12 +349,12 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
                 for (int r = 0; r < NR; r++) {
                     const C2<T> nd = csub<T>(PD[h][r], cadd<T>(cmul<T>(A, dL[r]), cmul<T>(C, dU[r])));
                     PD[h][r] = cmul<T>(nd, ib);
-                    tb[(2 + r) * RW + q * W + w] = PD[h][r];
+                    tb[(2 + r) * RW + slot(q)] = PD[h][r];
                 }
                 PA[h] = cmul<T>(cneg<T>(cmul<T>(A, aL)), ib);
                 PC[h] = cmul<T>(cneg<T>(cmul<T>(C, cU)), ib);
-                tb[q * W + w] = PA[h];
-                tb[RW + q * W + w] = PC[h];
+                tb[slot(q)] = PA[h];
+                tb[RW + slot(q)] = PC[h];
             }
         }
         cur ^= 1;
