@@ -420,10 +420,13 @@ class Server:
         self.h = h
         self._live = {}
 
-    def submit(self, dim, N, order, k, rhs, precision=PREC_F64, h=None):
+    def submit(self, dim, N, order, k, rhs, precision=PREC_F64, h=None, out=None):
+        """Queue one solve; `out` (optional, e.g. pinned memory for fast copies) receives the solution."""
         Nx, Ny, Nz = (N, N, N) if np.isscalar(N) else (tuple(N) + (1,))[:3]
         rhs = _c128(rhs)
-        sol = np.empty_like(rhs)
+        sol = np.empty_like(rhs) if out is None else out
+        if sol.dtype != np.complex128 or sol.size != rhs.size or not sol.flags.c_contiguous:
+            raise ValueError("out must be a contiguous complex128 array of the right-hand side's size")
         hx, hy, hz = (1.0 / (Nx + 1), 1.0 / (Ny + 1), 1.0 / (Nz + 1)) if h is None else h
         k = complex(k)
         r = _Request(dim, Nx, Ny, Nz if dim == 3 else 1, order, precision, k.real, k.imag, hx, hy, hz,
