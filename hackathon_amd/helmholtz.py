@@ -33,6 +33,16 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m hackathon_amd.build`")
+        # PyTorch ships its own HIP runtime under the same soname
+        # (libamdhip64.so.7) as the one libhh.so is linked against.  Loading
+        # PyTorch's first makes libhh.so bind to it, so that the process has
+        # one HIP runtime and the two share streams, events and memory.
+        try:
+            import torch
+            if torch.cuda.is_available():
+                torch.cuda.init()
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         L.hh_last_error.restype = ctypes.c_char_p
         L.hh_status_string.restype = ctypes.c_char_p

@@ -20,6 +20,15 @@ def say(msg):
     print(f"[case] {msg}", flush=True)
 
 
+@pytest.fixture(autouse=True, scope="module")
+def torch_runtime_first():
+    """PyTorch's HIP runtime has the soname of /opt/rocm's; it must be loaded
+    before the drop-in libraries so that the process holds one runtime."""
+    import torch
+    assert torch.cuda.is_available(), "GPU test run without a GPU"
+    torch.cuda.init()
+
+
 def dev(a):
     import torch
     return torch.from_numpy(np.ascontiguousarray(a, dtype=np.complex128)).cuda()
@@ -140,3 +149,81 @@ def test_multi_operator_plan_3d(oracle):
     for b, s in enumerate(setups):
         ref = oracle.solver3d(N, N, N, s["rhs"], oracle.lu3d(N, N, N, s["coef"]))
         assert relerr(sol[b], ref) <= 1e-12
+
+
+# 2D_Program/headers/structs.h: DSTN {int Nx, N; double coef}
+class DSTN(ctypes.Structure):
+    _fields_ = [("Nx", ctypes.c_int), ("N", ctypes.c_int), ("coef", ctypes.c_double)]
+
+
+def test_forward_reverse_dst_2d(oracle):
+    """forwardDST: DST-I of every row; reverseDST: row j of sol is the DST-I
+    of the mode-major column xhat[j + i*Ny] (2D_Program/solver_functions/DST.c:6-44)."""
+    say("drop-in forwardDST / reverseDST, Nx=63 Ny=40")
+    Nx, Ny = 63, 40
+    lib = ctypes.CDLL(os.path.join(PKG, "libhh_ref2d.so"))
+    rng = np.random.default_rng(3)
+    x = np.ascontiguousarray(rng.standard_normal((Ny, Nx)) + 1j * rng.standard_normal((Ny, Nx)))
+    sysv = System2()
+    sysv.lat.Nx, sysv.lat.Ny, sysv.lat.Nxy = Nx, Ny, Nx * Ny
+    dst = DSTN(Nx, 2 * Nx + 2, np.sqrt(2.0 / (Nx + 1)))
+    out = np.zeros_like(x)
+    null = ctypes.c_void_p(0)
+    lib.forwardDST(sysv, dst, _P(x.ctypes.data), _P(out.ctypes.data), null, null, null, null, null, null)
+    assert lib.hh_ref2d_last_status() == 0
+    ref = oracle.dst1_rows(x)
+    assert relerr(out, ref) <= 1e-13
+    xhat = np.ascontiguousarray(ref.T)          # mode-major: xhat[j + i*Ny]
+    back = np.zeros_like(x)
+    lib.reverseDST(sysv, dst, _P(xhat.ctypes.data), _P(back.ctypes.data), null, null, null, null, null, null)
+    assert lib.hh_ref2d_last_status() == 0
+    assert relerr(back, x) <= 1e-13               # DST-I is its own inverse
+
+
+def test_residual_2d_dropin(oracle):
+    """residual(System): res = A sol - rhs, error = sol - sol_analytic (residual.c:3-34)."""
+    say("drop-in residual 2D N=63 order 6")
+    N = 63
+    s = oracle.solve2d(6, N, 1, [20.0, 0.0, 3, 2, 0.5], threads=8)
+    lib = ctypes.CDLL(os.path.join(PKG, "libhh_ref2d.so"))
+    keep = {}
+    sysv = System2()
+    sysv.lat.Nx = sysv.lat.Ny = N
+    sysv.lat.Nxy = N * N
+    for name, arr in zip(("bp", "ap", "b", "a", "bm", "am"), s["coef"]):
+        keep[name] = np.ascontiguousarray(arr)
+        setattr(sysv, name, keep[name].ctypes.data)
+    rng = np.random.default_rng(9)
+    keep["sol"] = np.ascontiguousarray(s["sol"] + 1e-3 * (rng.standard_normal(N * N) + 1j * rng.standard_normal(N * N)))
+    keep["rhs"] = np.ascontiguousarray(s["rhs"])
+    keep["sol_analytic"] = np.ascontiguousarray(s["u"])
+    keep["res"] = np.zeros(N * N, np.complex128)
+    keep["error"] = np.zeros(N * N, np.complex128)
+    for name in ("sol", "rhs", "sol_analytic", "res", "error"):
+        setattr(sysv, name, keep[name].ctypes.data)
+    lib.residual(sysv)
+    assert lib.hh_ref2d_last_status() == 0
+    ref = oracle.residual2d(N, N, s["coef"], keep["sol"], keep["rhs"])
+    assert relerr(keep["res"], ref) <= 1e-12
+    assert np.array_equal(keep["error"], keep["sol"] - keep["sol_analytic"])
+
+
+def test_dst_layer_3d_dropin(oracle):
+    """3D DST(): 2D DST-I of a real Nx*Ny layer scaled 2/sqrt((Nx+1)(Ny+1))
+    (3D_Program/solver_functions/DST.c:148-204); the input is left unchanged."""
+    say("drop-in 3D DST layer 31 x 20")
+    Nx, Ny = 31, 20
+    lib = ctypes.CDLL(os.path.join(PKG, "libhh_ref3d.so"))
+    rng = np.random.default_rng(4)
+    b = np.ascontiguousarray(rng.standard_normal(Nx * Ny))
+    b0 = b.copy()
+    bhat = np.zeros_like(b)
+    null = ctypes.c_void_p(0)
+    lib.DST(ctypes.c_int(Nx), ctypes.c_int(Ny), _P(b.ctypes.data), _P(bhat.ctypes.data), null, null, null, null,
+            null, null)
+    assert lib.hh_ref3d_last_status() == 0
+    x = b.reshape(Ny, Nx).astype(np.complex128)
+    ref = oracle.dst1_rows(x)
+    ref = np.swapaxes(oracle.dst1_rows(np.ascontiguousarray(ref.T)), 0, 1)   # along y
+    assert relerr(bhat.reshape(Ny, Nx), ref.real) <= 1e-13
+    assert np.array_equal(b, b0)
