@@ -32,11 +32,23 @@ template <> struct tsel<float> { static constexpr int id = 1; };
 // TPB: target threads per block (256, or 64 for one-wave blocks whose
 // barriers are nearly free and whose small LDS footprint lets more blocks
 // share a CU)
+// LDS line stride (doubles) of a block's line buffers [re | im]: at least
+// 2 LP, and congruent to 32/LPB modulo 32 so that the tile loads and stores
+// of the strided kernels (lanes across the block's lines, then along them)
+// fall on distinct banks.
+constexpr int line_stride(int LP, int LPB) {
+    const int want = LPB >= 32 ? 1 : 32 / LPB;
+    int ls = 2 * LP;
+    while (ls % 32 != want % 32) ls++;
+    return ls;
+}
+
 template <int L, int TPB = 256> struct dst_cfg {
     static constexpr int P = fft_shape<L>::P;
     static constexpr int LPB = P >= TPB ? 1 : TPB / P;   // lines per block
     static constexpr int THREADS = P * LPB;
     static constexpr int LP = fft_shape<L>::LP;
+    static constexpr int LS = line_stride(LP, LPB);
 };
 
 // Position of element k of line `row` (the lines of every right-hand side
@@ -82,7 +94,7 @@ struct RowMap {
 // the LDS line buffers (element k of line l at position k+1), in tile-major
 // order so that each tile's LPB x tw elements are read as one contiguous
 // run.  Every load of a thread is issued before the first LDS write.
-template <typename Ti, typename T, int N, int LPB, int LP, int NT>
+template <typename Ti, typename T, int N, int LPB, int LP, int LS, int NT>
 HH_DEV void load_tiled_rows(T *smem, const C2<Ti> *in, const RowMap &map, long row0, int nr) {
     constexpr int LGL = LPB == 1 ? 0 : LPB == 2 ? 1 : LPB == 4 ? 2 : LPB == 8 ? 3 : LPB == 16 ? 4
                       : LPB == 32 ? 5 : LPB == 64 ? 6 : LPB == 128 ? 7 : 8;
@@ -102,7 +114,7 @@ HH_DEV void load_tiled_rows(T *smem, const C2<Ti> *in, const RowMap &map, long r
         const int l = rem >> lg, k = (tile << lg) + (rem & (tw - 1));
         const bool ok = e < slots && k < N && l < nr;
         v[i] = cconv<T, Ti>(in[ok ? map.next(l0, l).at(k) : l0.at(0)]);
-        pos[i] = ok ? (2 * l) * LP + lds_pad(k + 1) : -1;
+        pos[i] = ok ? l * LS + lds_pad(k + 1) : -1;
     }
 #pragma unroll
     for (int i = 0; i < ITER; i++) {
@@ -117,12 +129,13 @@ template <typename Ti, typename T, typename To, int L, int TPB, bool TIN>
 __global__ __launch_bounds__((dst_cfg<L, TPB>::THREADS)) void k_dst_fft_rows(
     const C2<Ti> *in, C2<To> *out, long nrows, T scale,
     const C2<T> *__restrict__ tw, RowMap min_, RowMap mout) {
-    constexpr int P = dst_cfg<L, TPB>::P, LPB = dst_cfg<L, TPB>::LPB, LP = dst_cfg<L, TPB>::LP;
+    constexpr int P = dst_cfg<L, TPB>::P, LPB = dst_cfg<L, TPB>::LPB, LP = dst_cfg<L, TPB>::LP,
+                  LS = dst_cfg<L, TPB>::LS;
     constexpr int M = L / 2, N = M - 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *smem = reinterpret_cast<T *>(smem_raw);
     const int li = threadIdx.x / P, t = threadIdx.x % P;
-    T *sre = smem + (2 * li) * LP, *sim = smem + (2 * li + 1) * LP;
+    T *sre = smem + li * LS, *sim = smem + li * LS + LP;
     const long row = (long)blockIdx.x * LPB + li;
     const bool valid = row < nrows;
     const LinePos lout = mout.line(valid ? row : 0);
@@ -130,7 +143,7 @@ __global__ __launch_bounds__((dst_cfg<L, TPB>::THREADS)) void k_dst_fft_rows(
         constexpr int NT = dst_cfg<L, TPB>::THREADS;
         const long row0 = (long)blockIdx.x * LPB;
         const long nr = nrows - row0 < LPB ? nrows - row0 : LPB;
-        load_tiled_rows<Ti, T, N, LPB, LP, NT>(smem, in, min_, row0, (int)nr);
+        load_tiled_rows<Ti, T, N, LPB, LP, LS, NT>(smem, in, min_, row0, (int)nr);
         __syncthreads();
     }
     // Branch-free: every load is issued (index clamped into the row) and
@@ -155,7 +168,7 @@ __global__ __launch_bounds__((dst_cfg<L, TPB>::THREADS)) void k_dst_fft_rows(
 // line buffers (element q of line c at position q+1).  Every load of a
 // thread is issued before the first LDS write so that they overlap; lanes
 // past the end of the tile read the tile's first element and write nothing.
-template <typename Ti, typename T, int N, int LPB, int LP, int NT>
+template <typename Ti, typename T, int N, int LPB, int LP, int LS, int NT>
 HH_DEV void load_col_tile(T *smem, const C2<Ti> *in, long base, long c0, long ncols, long elem_stride) {
     constexpr int ITER = (N * LPB + NT - 1) / NT;
     C2<T> v[ITER];
@@ -172,14 +185,14 @@ HH_DEV void load_col_tile(T *smem, const C2<Ti> *in, long base, long c0, long nc
         const int e = threadIdx.x + i * NT;
         if (e < N * LPB) {
             const int c = e % LPB, q = e / LPB;
-            smem[(2 * c) * LP + lds_pad(q + 1)] = v[i].x;
-            smem[(2 * c + 1) * LP + lds_pad(q + 1)] = v[i].y;
+            smem[c * LS + lds_pad(q + 1)] = v[i].x;
+            smem[c * LS + LP + lds_pad(q + 1)] = v[i].y;
         }
     }
 }
 
 // Store of the tile back to its strided lines (positions 1..N of each line).
-template <typename T, typename To, int N, int LPB, int LP, int NT>
+template <typename T, typename To, int N, int LPB, int LP, int LS, int NT>
 HH_DEV void store_col_tile(const T *smem, C2<To> *out, long base, long c0, long ncols, long elem_stride) {
     constexpr int ITER = (N * LPB + NT - 1) / NT;
 #pragma unroll
@@ -188,7 +201,7 @@ HH_DEV void store_col_tile(const T *smem, C2<To> *out, long base, long c0, long 
         const int c = e % LPB, q = e / LPB;
         if (e < N * LPB && c0 + c < ncols)
             out[base + c + (long)q * elem_stride] =
-                cconv<To, T>(mk<T>(smem[(2 * c) * LP + lds_pad(q + 1)], smem[(2 * c + 1) * LP + lds_pad(q + 1)]));
+                cconv<To, T>(mk<T>(smem[c * LS + lds_pad(q + 1)], smem[c * LS + LP + lds_pad(q + 1)]));
     }
 }
 
@@ -196,7 +209,8 @@ template <typename Ti, typename T, typename To, int L, int TPB>
 __global__ __launch_bounds__((dst_cfg<L, TPB>::THREADS)) void k_dst_fft_cols(
     const C2<Ti> *in, C2<To> *out, long ncols, long ntiles,
     long plane_stride, long elem_stride, T scale, const C2<T> *__restrict__ tw) {
-    constexpr int P = dst_cfg<L, TPB>::P, LPB = dst_cfg<L, TPB>::LPB, LP = dst_cfg<L, TPB>::LP;
+    constexpr int P = dst_cfg<L, TPB>::P, LPB = dst_cfg<L, TPB>::LPB, LP = dst_cfg<L, TPB>::LP,
+                  LS = dst_cfg<L, TPB>::LS;
     constexpr int M = L / 2, N = M - 1, NT = dst_cfg<L, TPB>::THREADS;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *smem = reinterpret_cast<T *>(smem_raw);
@@ -204,10 +218,10 @@ __global__ __launch_bounds__((dst_cfg<L, TPB>::THREADS)) void k_dst_fft_cols(
     const long c0 = tile * LPB;
     const long base = plane * plane_stride + c0;
     // coalesced tile load: lanes run along the columns of one row
-    load_col_tile<Ti, T, N, LPB, LP, NT>(smem, in, base, c0, ncols, elem_stride);
+    load_col_tile<Ti, T, N, LPB, LP, LS, NT>(smem, in, base, c0, ncols, elem_stride);
     __syncthreads();
     const int li = threadIdx.x / P, t = threadIdx.x % P;
-    T *sre = smem + (2 * li) * LP, *sim = smem + (2 * li + 1) * LP;
+    T *sre = smem + li * LS, *sim = smem + li * LS + LP;
     auto get = [&](int n) -> C2<T> {
         if (n == 0 || n == M) return mk<T>(0, 0);
         if (n < M) return mk<T>(sre[lds_pad(n)], sim[lds_pad(n)]);
@@ -221,7 +235,7 @@ __global__ __launch_bounds__((dst_cfg<L, TPB>::THREADS)) void k_dst_fft_cols(
     };
     fft_line<T, L, true>(sre, sim, tw, t, get, put);
     __syncthreads();
-    store_col_tile<T, To, N, LPB, LP, NT>(smem, out, base, c0, ncols, elem_stride);
+    store_col_tile<T, To, N, LPB, LP, LS, NT>(smem, out, base, c0, ncols, elem_stride);
 }
 
 // ------------------------------------------------------------------------
@@ -242,6 +256,7 @@ template <int M, int TPB = 256> struct half_cfg {
     static constexpr int LPB = P >= TPB ? 1 : TPB / P;
     static constexpr int THREADS = P * LPB;
     static constexpr int LP = fft_shape<M>::LP;
+    static constexpr int LS = line_stride(LP, LPB);
     static constexpr int K = (M / 2) / P;   // pairs (F_k, F_{M-k}) per thread, M >= 2P
 };
 
@@ -309,19 +324,19 @@ __global__ __launch_bounds__((half_cfg<M, TPB>::THREADS)) void k_dst_half_rows(c
                                                                             const C2<T> *__restrict__ aux,
                                                                             RowMap min_, RowMap mout) {
     using cfg = half_cfg<M, TPB>;
-    constexpr int P = cfg::P, LPB = cfg::LPB, LP = cfg::LP, N = M - 1, NT = cfg::THREADS;
+    constexpr int P = cfg::P, LPB = cfg::LPB, LP = cfg::LP, LS = cfg::LS, N = M - 1, NT = cfg::THREADS;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *smem = reinterpret_cast<T *>(smem_raw);
-    T *tot = smem + 2 * LPB * LP;   // per line: 2 * (P/64 + 1) reals
+    T *tot = smem + LPB * LS;   // per line: 2 * (P/64 + 1) reals
     const int li = threadIdx.x / P, t = threadIdx.x % P;
-    T *sre = smem + (2 * li) * LP, *sim = smem + (2 * li + 1) * LP;
+    T *sre = smem + li * LS, *sim = smem + li * LS + LP;
     const long row0 = (long)blockIdx.x * LPB;
     const long row = row0 + li;
     const bool valid = row < nrows;
     const T *sn = reinterpret_cast<const T *>(aux + M);
     const long nr = nrows - row0 < LPB ? nrows - row0 : LPB;
     if constexpr (TIN) {   // tiled input: stage the block's lines through LDS
-        load_tiled_rows<Ti, T, N, LPB, LP, NT>(smem, in, min_, row0, (int)nr);
+        load_tiled_rows<Ti, T, N, LPB, LP, LS, NT>(smem, in, min_, row0, (int)nr);
         __syncthreads();
     }
     // branch-free (clamped indices, zero selected for n = 0) so that all
@@ -358,7 +373,7 @@ __global__ __launch_bounds__((half_cfg<M, TPB>::THREADS)) void k_dst_half_rows(c
             const int e = threadIdx.x + i * NT;
             if (e < nr * N) {
                 const int l = e / N, m = e % N + 1;
-                y[e] = cconv<To, T>(mk<T>(smem[(2 * l) * LP + lds_pad(m)], smem[(2 * l + 1) * LP + lds_pad(m)]));
+                y[e] = cconv<To, T>(mk<T>(smem[l * LS + lds_pad(m)], smem[l * LS + LP + lds_pad(m)]));
             }
         }
     } else {
@@ -377,7 +392,7 @@ __global__ __launch_bounds__((half_cfg<M, TPB>::THREADS)) void k_dst_half_rows(c
             const int l = rem >> lg, k = (tile << lg) + (rem & (tw - 1));
             if (k < N && l < nr)
                 out[mout.next(l0, l).at(k)] =
-                    cconv<To, T>(mk<T>(smem[(2 * l) * LP + lds_pad(k + 1)], smem[(2 * l + 1) * LP + lds_pad(k + 1)]));
+                    cconv<To, T>(mk<T>(smem[l * LS + lds_pad(k + 1)], smem[l * LS + LP + lds_pad(k + 1)]));
         }
     }
 }
@@ -388,17 +403,17 @@ __global__ __launch_bounds__((half_cfg<M, TPB>::THREADS)) void k_dst_half_cols(c
                                                                             long elem_stride, T scale,
                                                                             const C2<T> *__restrict__ aux) {
     using cfg = half_cfg<M, TPB>;
-    constexpr int P = cfg::P, LPB = cfg::LPB, LP = cfg::LP, N = M - 1, NT = cfg::THREADS;
+    constexpr int P = cfg::P, LPB = cfg::LPB, LP = cfg::LP, LS = cfg::LS, N = M - 1, NT = cfg::THREADS;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *smem = reinterpret_cast<T *>(smem_raw);
-    T *tot = smem + 2 * LPB * LP;
+    T *tot = smem + LPB * LS;
     const long tile = blockIdx.x % ntiles, plane = blockIdx.x / ntiles;
     const long c0 = tile * LPB;
     const long base = plane * plane_stride + c0;
-    load_col_tile<Ti, T, N, LPB, LP, NT>(smem, in, base, c0, ncols, elem_stride);
+    load_col_tile<Ti, T, N, LPB, LP, LS, NT>(smem, in, base, c0, ncols, elem_stride);
     __syncthreads();
     const int li = threadIdx.x / P, t = threadIdx.x % P;
-    T *sre = smem + (2 * li) * LP, *sim = smem + (2 * li + 1) * LP;
+    T *sre = smem + li * LS, *sim = smem + li * LS + LP;
     const T *sn = reinterpret_cast<const T *>(aux + M);
     auto get = [&](int n) -> C2<T> {
         if (n == 0) return mk<T>(0, 0);
@@ -416,7 +431,7 @@ __global__ __launch_bounds__((half_cfg<M, TPB>::THREADS)) void k_dst_half_cols(c
     __syncthreads();
     constexpr int NW = P > 64 ? P / 64 : 1;
     half_post<T, M>(sre, sim, tot + li * 2 * NW, tot + li * 2 * NW + NW, t, scale);
-    store_col_tile<T, To, N, LPB, LP, NT>(smem, out, base, c0, ncols, elem_stride);
+    store_col_tile<T, To, N, LPB, LP, LS, NT>(smem, out, base, c0, ncols, elem_stride);
 }
 
 // Dense DST: out[line][k] = sum_n S[k][n] in[line][n].  Block tile: 32 lines x
@@ -486,7 +501,7 @@ bool fft_supported(int N) {
 template <typename Ti, typename T, typename To, int L, int TPB>
 static hipError_t launch_fft_L(const Lines &ln, const void *in, void *out, const void *tw, hipStream_t st) {
     using cfg = dst_cfg<L, TPB>;
-    size_t shm = sizeof(T) * 2 * cfg::LPB * cfg::LP;
+    size_t shm = sizeof(T) * cfg::LPB * cfg::LS;
     T scale = (T)sqrt(0.5 / (double)(L / 2));
     const C2<Ti> *pi = (const C2<Ti> *)in;
     C2<To> *po = (C2<To> *)out;
@@ -522,7 +537,7 @@ template <typename Ti, typename T, typename To, int M, int TPB>
 static hipError_t launch_half_M(const Lines &ln, const void *in, void *out, const void *aux, hipStream_t st) {
     using cfg = half_cfg<M, TPB>;
     constexpr int NW = cfg::P > 64 ? cfg::P / 64 : 1;
-    size_t shm = sizeof(T) * (2 * cfg::LPB * cfg::LP + 2 * cfg::LPB * NW);
+    size_t shm = sizeof(T) * (cfg::LPB * cfg::LS + 2 * cfg::LPB * NW);
     T scale = (T)sqrt(2.0 / (double)M);
     const C2<Ti> *pi = (const C2<Ti> *)in;
     C2<To> *po = (C2<To> *)out;
