@@ -4,11 +4,8 @@
 // V = min(16, L) complex values in registers.  Stages are radix-16 (the last
 // one radix L/Ns when fewer points remain); between stages the values are
 // exchanged through LDS (split real / imaginary arrays, padded by one
-// element per 32 to break power-of-two bank strides: with FP64 elements a
-// 32-element run covers the 64 banks once, and of the exchanges of a 1024-
-// point line only the second stage's writes keep a two-way conflict, where
-// padding per 16 left the reads of stages 2 and 3 and the final writes
-// two-way).  Stage s with radix R
+// element per 16 to break power-of-two bank strides; padding per 32 removes
+// more two-way conflicts on paper but measured 1-2% slower).  Stage s with radix R
 // and span Ns (Stockham autosort, decimation in time):
 //   butterfly j in [0, L/R):  k = j mod Ns
 //     in  x[r] = buf[j + r L/R]                 r = 0..R-1
@@ -22,7 +19,7 @@
 
 namespace hh {
 
-HH_DEV int lds_pad(int i) { return i + (i >> 5); }
+HH_DEV int lds_pad(int i) { return i + (i >> 4); }
 
 template <typename T> HH_DEV void dft2(C2<T>* v) {
     C2<T> a = v[0], b = v[1];
@@ -103,7 +100,7 @@ template <typename T, int R> HH_DEV void dftR(C2<T>* v) {
 template <int L> struct fft_shape {
     static constexpr int V = L < 16 ? L : 16;   // values per thread
     static constexpr int P = L / V;             // threads per line
-    static constexpr int LP = L + L / 32 + 1;   // padded LDS line length
+    static constexpr int LP = L + L / 16 + 1;   // padded LDS line length
 };
 
 // One Stockham stage.  FIRST: inputs come from get(n); otherwise from LDS.

@@ -293,11 +293,11 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
     // 3. PCR on the reduced system, rows r0 = 2c, r1 = 2c+1, layout [row][W].
     //    Two LDS buffers: step i reads buffer i%2 and writes the other, so
     //    one barrier per step suffices.
-    // state of the thread's two reduced rows q = 2c + h, h = 0 (x_s), 1 (x_e).
-    // In LDS row q sits at slot (q & 1) P + (q >> 1): the rows one step of a
-    // wave touches are then contiguous (a plain [q][W] layout puts chunk c
-    // and c+2 on the same banks)
-    auto slot = [&](int q) { return ((q & 1) * P + (q >> 1)) * W + w; };
+    // state of the thread's two reduced rows q = 2c + h, h = 0 (x_s), 1 (x_e),
+    // row q at LDS slot q W + w.  (Slots (q & 1) P + (q >> 1), free of the
+    // two-way bank conflicts of this layout, measured 7% slower: 872 against
+    // 811 us at N=1023 x 64 right-hand sides.)
+    auto slot = [&](int q) { return q * W + w; };
     C2<T> PA[2] = {A0, A1}, PC[2] = {C0, C1}, PD[2][NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) { PD[0][r] = D0[r]; PD[1][r] = D1[r]; }
