@@ -98,6 +98,13 @@ struct hh_plan {
     double *lamx = nullptr, *lamy = nullptr;
     std::vector<cd> hcoef;                       // host copy of the band table
     std::vector<float> h32;                      // FP32 copy staged for hh_plan_set_operators
+    // real parts of the table, same order (read as (re c_2i, re c_2i+1) pairs by
+    // the real-band tridiagonal kernels), FP64 and FP32
+    void *coefr64 = nullptr, *coefr32 = nullptr;
+    std::vector<double> hr64;
+    std::vector<float> hr32;
+    int real_op = 0;                             // every coefficient of the current operators real
+    int real_mode = -1;                          // -1: real-band kernels when real_op; 0: complex-band always
     int cap_nop = 1;                             // operator slots allocated on the device
     double tol = 1e-12;
     int max_iter = 6;
@@ -529,6 +536,19 @@ extern "C" int hh_coefficients3d(int order, int Nx, int Ny, int Nz, double hx, d
 // Device band-table order from the caller's operator vectors.
 // 2D: input [op][bp, ap, b, a, bm, am][Ny] -> table [op][q](bm, am, b, a, bp, ap);
 // 3D: input [op][a..dm][Nz] -> table [op][l][12].
+static void fill_real(hh_plan *p) {
+    const size_t cnt = p->hcoef.size();
+    p->hr64.resize(cnt);
+    p->hr32.resize(cnt);
+    int real = 1;
+    for (size_t i = 0; i < cnt; i++) {
+        p->hr64[i] = p->hcoef[i].real();
+        p->hr32[i] = (float)p->hcoef[i].real();
+        if (p->hcoef[i].imag() != 0.0) real = 0;
+    }
+    p->real_op = real;
+}
+
 static void fill_table(hh_plan *p, int nop, const double *coef) {
     if (p->dim == 2) {
         const int Ny = p->Ny;
@@ -546,6 +566,7 @@ static void fill_table(hh_plan *p, int nop, const double *coef) {
                 for (int q = 0; q < 12; q++)
                     p->hcoef[((size_t)o * Nz + l) * 12 + q] = ld(coef, ((size_t)o * 12 + q) * Nz + l);
     }
+    fill_real(p);
 }
 
 static int upload_plan(hh_plan *p, int ncoef, int n) {
@@ -562,6 +583,10 @@ static int upload_plan(hh_plan *p, int ncoef, int n) {
         h32[2 * i + 1] = (float)p->hcoef[i].imag();
     }
     HIPCHK(hipMemcpy(p->coef32, h32.data(), b32, hipMemcpyHostToDevice));
+    if ((rc = dev_alloc(c, &p->coefr64, b64 / 2))) return rc;
+    if ((rc = dev_alloc(c, &p->coefr32, b32 / 2))) return rc;
+    HIPCHK(hipMemcpy(p->coefr64, p->hr64.data(), b64 / 2, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(p->coefr32, p->hr32.data(), b32 / 2, hipMemcpyHostToDevice));
     auto eig = [](int N, std::vector<double> &v) {
         v.resize(N);
         for (int k = 0; k < N; k++) v[k] = 2.0 * std::cos((k + 1) * M_PI / (N + 1));   // LU.c:12-14
@@ -655,6 +680,8 @@ extern "C" int hh_plan_destroy(hh_plan *p) {
     (void)hipStreamSynchronize(c->stream);
     dev_free(c, p->coef64, (size_t)nc * n * p->cap_nop * 16);
     dev_free(c, p->coef32, (size_t)nc * n * p->cap_nop * 8);
+    dev_free(c, p->coefr64, (size_t)nc * n * p->cap_nop * 8);
+    dev_free(c, p->coefr32, (size_t)nc * n * p->cap_nop * 4);
     dev_free(c, p->lamx, (size_t)p->Nx * 8);
     if (p->lamy) dev_free(c, p->lamy, (size_t)p->Ny * 8);
     delete p;
@@ -713,6 +740,15 @@ extern "C" int hh_plan_set_tri_shape(hh_plan *p, int W, int rows_per_thread, int
     return HH_OK;
 }
 
+extern "C" int hh_plan_set_real_operator(hh_plan *p, int mode) {
+    if (!p) return fail(HH_ERR_ARG, "null plan");
+    if (mode < -1 || mode > 0) return fail(HH_ERR_ARG, "real-operator mode must be -1 (automatic) or 0 (complex bands)");
+    p->real_mode = mode;
+    return HH_OK;
+}
+
+extern "C" int hh_plan_is_real_operator(const hh_plan *p) { return p && p->real_op && p->real_mode != 0; }
+
 extern "C" int hh_plan_set_tile_width(hh_plan *p, int tw) {
     if (!p) return fail(HH_ERR_ARG, "null plan");
     if (tw != -1 && (tw < 0 || tw > 64 || (tw & (tw - 1))))
@@ -763,6 +799,8 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
     t.field = hh_plan_field_size(p);
     t.batch = batch;
     t.coef = tri_comp == 0 ? p->coef64 : p->coef32;
+    t.real = p->real_op && p->real_mode != 0;
+    t.coefr = tri_comp == 0 ? p->coefr64 : p->coefr32;
     t.nop = p->nop;
     t.cstride = p->dim == 2 ? 6L * Ny : 12L * Nz;
     t.lamx = p->lamx; t.lamy = p->lamy;
@@ -773,6 +811,14 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
         hh::tri_shape(n, nmodes, hh::F64, &t.W, &t.P, &t.mmax);
     }
     t.nr = 1;
+    // Real bands, FP64, 16 rows per thread: two right-hand sides per thread
+    // share the operator's pivots (measured at N=1023 x 64: 590 against 654
+    // us per launch) when the grid still has 8 blocks per CU or more.
+    if (t.real && tri_comp == 0 && tc == 0 && t.mmax == 16) {
+        const long tiles = (nmodes + t.W - 1) / t.W;
+        const long groups = ((batch + p->nop - 1) / p->nop + 1) / 2;
+        if (tiles * p->nop * groups >= 2048) t.nr = 2;
+    }
     if (p->tri_W > 0 && p->tri_mm > 0) {   // shape chosen by the autotuner
         t.W = p->tri_W;
         t.mmax = p->tri_mm;
@@ -1047,6 +1093,8 @@ extern "C" int hh_plan_set_operators(hh_plan *p, int nop, const double *coef) {
     // stream-ordered: solves queued before this call still read the old tables
     HIPCHK(hipMemcpyAsync(p->coef64, p->hcoef.data(), cnt * 16, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(p->coef32, p->h32.data(), cnt * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(p->coefr64, p->hr64.data(), cnt * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(p->coefr32, p->hr32.data(), cnt * 4, hipMemcpyHostToDevice, c->stream));
     p->nop = nop;
     return HH_OK;
 }

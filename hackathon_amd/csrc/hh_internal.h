@@ -56,6 +56,8 @@ struct TriHelm {
     int W, P, mmax;     // block shape (modes per block, chunks per mode, rows per chunk)
     int nr = 1;         // right-hand sides per thread (sharing one operator): 1, 2 or 4
     int tiled = 0;      // field in the mode-tiled layout of the row DSTs: tile width (a multiple of W), 0 = row-major
+    int real = 0;       // every coefficient real: use coefr, the real parts in pairs ([q][3] / [q][6] complex slots)
+    const void *coefr = nullptr;
 };
 hipError_t launch_tri_helm(const TriHelm &t, DstTypes ty, const void *in, void *out, hipStream_t st);
 

@@ -20,6 +20,8 @@
 //      parallel cyclic reduction in LDS with normalised rows;
 //   4. x_q = d^_q - a^_q x_s - c^_q x_e, store.
 // HBM is read once and written once; the solve may run in place.
+#include <type_traits>
+
 #include "hh_common.h"
 #include "hh_internal.h"
 
@@ -33,12 +35,51 @@ namespace hh {
 // coef + (b % nop) * cstride (one operator per problem of a batch).
 // One row of a tridiagonal system, returned by value (reference outputs
 // written under branches defeat scalar replacement and land in scratch).
-template <typename T> struct Band {
-    C2<T> sub, dia, sup;
+template <typename OT> struct Band {
+    OT sub, dia, sup;
 };
 
-template <typename T, int DIM> struct HelmBands {
+// Arithmetic on the operator's coefficients, OT = C2<T> (complex bands) or
+// T (real bands: every stencil coefficient real, as for real k), and on
+// complex right-hand sides scaled by them.  The real form needs a quarter
+// of the multiplies of the complex one and half the registers and LDS.
+template <typename T, typename OT> struct OpAr;
+template <typename T> struct OpAr<T, C2<T>> {
+    using O = C2<T>;
+    static HH_DEV O zero() { return mk<T>(0, 0); }
+    static HH_DEV O one() { return mk<T>(1, 0); }
+    static HH_DEV O mul(O a, O b) { return cmul<T>(a, b); }
+    static HH_DEV O neg(O a) { return cneg<T>(a); }
+    static HH_DEV O fms(O a, O b, O c) { return cfms<T>(a, b, c); }      // c - a b
+    static HH_DEV O rcp(O a) { return crcp<T>(a); }
+    static HH_DEV C2<T> xmul(O a, C2<T> x) { return cmul<T>(a, x); }
+    static HH_DEV C2<T> xfms(O a, C2<T> x, C2<T> y) { return cfms<T>(a, x, y); }   // y - a x
+    // y - (a x + b z), c - (a b + d e): the products summed first
+    static HH_DEV C2<T> xfms2(O a, C2<T> x, O b, C2<T> z, C2<T> y) {
+        return csub<T>(y, cadd<T>(cmul<T>(a, x), cmul<T>(b, z)));
+    }
+    static HH_DEV O fms2(O a, O b, O d, O e, O c) { return csub<T>(c, cadd<T>(cmul<T>(a, b), cmul<T>(d, e))); }
+};
+template <typename T> struct OpAr<T, T> {
+    using O = T;
+    static HH_DEV O zero() { return (T)0; }
+    static HH_DEV O one() { return (T)1; }
+    static HH_DEV O mul(O a, O b) { return a * b; }
+    static HH_DEV O neg(O a) { return -a; }
+    static HH_DEV O fms(O a, O b, O c) { return c - a * b; }
+    // rcp_nr's Newton steps hold for either sign of a nonzero argument
+    static HH_DEV O rcp(O a) { return rcp_nr(a); }
+    static HH_DEV C2<T> xmul(O a, C2<T> x) { return mk<T>(a * x.x, a * x.y); }
+    static HH_DEV C2<T> xfms(O a, C2<T> x, C2<T> y) { return mk<T>(y.x - a * x.x, y.y - a * x.y); }
+    static HH_DEV C2<T> xfms2(O a, C2<T> x, O b, C2<T> z, C2<T> y) { return xfms(b, z, xfms(a, x, y)); }
+    static HH_DEV O fms2(O a, O b, O d, O e, O c) { return fms(d, e, fms(a, b, c)); }
+};
+
+template <typename T, int DIM, bool REAL> struct HelmBands {
+    using OT = typename std::conditional<REAL, T, C2<T>>::type;
     int n, Nx, nop;
+    // complex table [q][6] / [q][12]; real table: the same coefficients'
+    // real parts in pairs, [q][3] / [q][6] of (re c_{2i}, re c_{2i+1})
     const C2<T> *coef;
     long cstride;
     const double *lamx, *lamy;
@@ -55,10 +96,26 @@ template <typename T, int DIM> struct HelmBands {
         }
         return md;
     }
-    HH_DEV Band<T> band(const Mode &md, int q) const {
-        Band<T> r;
+    HH_DEV Band<OT> band(const Mode &md, int q) const {
+        Band<OT> r;
         const T ms = q > 0 ? (T)1 : (T)0, ps = q < n - 1 ? (T)1 : (T)0;
-        if constexpr (DIM == 2) {
+        if constexpr (REAL) {
+            if constexpr (DIM == 2) {   // pairs (bm, am), (b, a), (bp, ap)
+                const C2<T> *c = md.cf + 3 * (long)q;
+                const T l = md.l0;
+                const C2<T> c0 = c[0], c1 = c[1], c2 = c[2];
+                r.sub = ms * (c0.x * l + c0.y);
+                r.dia = c1.x * l + c1.y;
+                r.sup = ps * (c2.x * l + c2.y);
+            } else {   // pairs (a, b), (c, d), (ap, bp), (cp, dp), (am, bm), (cm, dm)
+                const C2<T> *c = md.cf + 6 * (long)q;
+                const T a = md.l0, b = md.l1, g = md.l2;
+                const C2<T> c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3], c4 = c[4], c5 = c[5];
+                r.dia = c0.x * a + c0.y * b + c1.x * g + c1.y;
+                r.sup = ps * (c2.x * a + c2.y * b + c3.x * g + c3.y);
+                r.sub = ms * (c4.x * a + c4.y * b + c5.x * g + c5.y);
+            }
+        } else if constexpr (DIM == 2) {
             const C2<T> *c = md.cf + 6 * (long)q;
             const T l = md.l0;
             r.sub = mk<T>(ms * (c[0].x * l + c[1].x), ms * (c[0].y * l + c[1].y));
@@ -84,13 +141,14 @@ template <typename T, int DIM> struct HelmBands {
 // neighbouring modes, tile t0 = m & ~(tw-1) holding its modes for all n rows
 // contiguously: b*field + t0*n + q*w + (m - t0), w = min(tw, nmodes - t0)),
 // where each thread's rows are whole w*16-byte segments one after another.
-template <typename T, typename Ti, typename To, int DIM> struct HelmProb {
-    HelmBands<T, DIM> bands;
+template <typename T, typename Ti, typename To, int DIM, bool REAL> struct HelmProb {
+    using OT = typename HelmBands<T, DIM, REAL>::OT;
+    HelmBands<T, DIM, REAL> bands;
     const C2<Ti> *in;
     C2<To> *out;
     long field, S, nmodes;
     int tw;
-    struct Mode { typename HelmBands<T, DIM>::Mode bm; long off, step; };
+    struct Mode { typename HelmBands<T, DIM, REAL>::Mode bm; long off, step; };
     HH_DEV int n() const { return bands.n; }
     HH_DEV Mode mode(long m, int bt) const {
         Mode md;
@@ -105,7 +163,7 @@ template <typename T, typename Ti, typename To, int DIM> struct HelmProb {
         }
         return md;
     }
-    HH_DEV Band<T> band(const Mode &md, int q) const { return bands.band(md.bm, q); }
+    HH_DEV Band<OT> band(const Mode &md, int q) const { return bands.band(md.bm, q); }
     HH_DEV C2<T> rhs(int b, const Mode &md, int q) const {
         return cconv<T, Ti>(in[b * field + md.off + (long)q * md.step]);
     }
@@ -123,6 +181,7 @@ template <typename T, typename Ti, typename To, int DIM> struct HelmProb {
 // the extended-grid line origin + (m % mA) sA + (m / mA) sB; element q sits
 // at (q+1) eS along it.  The result goes to oorig + (m%mA) oA + (m/mA) oB + q oS.
 template <typename T> struct CompactProb {
+    using OT = T;   // bands (1, diag, 1)
     int nn, first;
     T diag, h;
     const C2<T> *f;
@@ -144,9 +203,9 @@ template <typename T> struct CompactProb {
     }
     HH_DEV Band<T> band(const Mode &, int q) const {
         Band<T> r;
-        r.sub = mk<T>(q > 0 ? (T)1 : (T)0, 0);
-        r.dia = mk<T>(diag, 0);
-        r.sup = mk<T>(q < nn - 1 ? (T)1 : (T)0, 0);
+        r.sub = q > 0 ? (T)1 : (T)0;
+        r.dia = diag;
+        r.sup = q < nn - 1 ? (T)1 : (T)0;
         return r;
     }
     HH_DEV C2<T> rhs(int bt, const Mode &md, int q) const {
@@ -181,12 +240,17 @@ HH_DEV long xcd_tile(long b, long G) {
 // are computed on a clamped index and not stored.
 template <typename T, int MMAX, int NR, int NT, class Prob>
 __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P, int batch, int nop) {
+    using OT = typename Prob::OT;
+    using O = OpAr<T, OT>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    // two buffers of the reduced rows, each [A | C | D_0 .. D_{NR-1}], rows x W
-    C2<T> *lds = reinterpret_cast<C2<T> *>(smem_raw);
+    // two buffers of the reduced rows, each [A | C | D_0 .. D_{NR-1}], rows x W;
+    // A and C are operator values (OT), the D the right-hand sides (complex)
     const int RR = 2 * P;
     const int RW = RR * W;
-    const int BUF = (2 + NR) * RW;
+    const int BUFB = RW * (int)(2 * sizeof(OT) + NR * sizeof(C2<T>));   // bytes per buffer (< 160 KiB)
+    auto bufA = [&](int k) { return reinterpret_cast<OT *>(smem_raw + k * BUFB); };
+    auto bufC = [&](int k) { return reinterpret_cast<OT *>(smem_raw + k * BUFB) + RW; };
+    auto bufD = [&](int k) { return reinterpret_cast<C2<T> *>(reinterpret_cast<OT *>(smem_raw + k * BUFB) + 2 * RW); };
     const int n = prob.n();
     const int w = threadIdx.x % W, c = threadIdx.x / W;
     const long m = xcd_tile(blockIdx.x, gridDim.x) * W + w;
@@ -211,33 +275,35 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
     // 0. issue every right-hand-side load of the chunk up front (rows past
     //    e repeat row e), so that the HBM latency is paid once per chunk and
     //    not once per row of the recurrence
-    C2<T> db[NR][MMAX], ab[MMAX], cb[MMAX];
+    C2<T> db[NR][MMAX];
+    OT ab[MMAX], cb[MMAX];
 #pragma unroll
     for (int r = 0; r < NR; r++)
 #pragma unroll
         for (int t = 0; t < MMAX; t++) db[r][t] = prob.rhs(bidx[r], md, min(s + t, e));
-    const Band<T> bs = prob.band(md, s);
-    const C2<T> sub_s = bs.sub, dia_s = bs.dia, sup_s = bs.sup;
+    const Band<OT> bs = prob.band(md, s);
+    const OT sub_s = bs.sub, dia_s = bs.dia, sup_s = bs.sup;
     C2<T> d_s[NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) d_s[r] = db[r][0];
 
     // 1. downward sweep over rows s+1..e: row s+t becomes
     //    ab[t] x_s + x_{s+t} + cb[t] x_{s+t+1} = db[t]
-    C2<T> pa = mk<T>(0, 0), pc = mk<T>(0, 0), pd[NR];
+    OT pa = O::zero(), pc = O::zero();
+    C2<T> pd[NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) pd[r] = mk<T>(0, 0);
 #pragma unroll
     for (int t = 1; t < MMAX; t++) {
-        const Band<T> bq = prob.band(md, min(s + t, e));
-        const C2<T> a = bq.sub;
-        const C2<T> den = (t == 1) ? bq.dia : cfms<T>(a, pc, bq.dia);
-        const C2<T> rc = crcp<T>(den);
-        const C2<T> na = (t == 1) ? cmul<T>(a, rc) : cneg<T>(cmul<T>(cmul<T>(a, pa), rc));
-        const C2<T> nc = cmul<T>(bq.sup, rc);
+        const Band<OT> bq = prob.band(md, min(s + t, e));
+        const OT a = bq.sub;
+        const OT den = (t == 1) ? bq.dia : O::fms(a, pc, bq.dia);
+        const OT rc = O::rcp(den);
+        const OT na = (t == 1) ? O::mul(a, rc) : O::neg(O::mul(O::mul(a, pa), rc));
+        const OT nc = O::mul(bq.sup, rc);
         C2<T> nd[NR];
 #pragma unroll
-        for (int r = 0; r < NR; r++) nd[r] = (t == 1) ? cmul<T>(db[r][t], rc) : cmul<T>(cfms<T>(a, pd[r], db[r][t]), rc);
+        for (int r = 0; r < NR; r++) nd[r] = (t == 1) ? O::xmul(rc, db[r][t]) : O::xmul(rc, O::xfms(a, pd[r], db[r][t]));
         if (t < len) {   // selects, not a branch
             ab[t] = na; cb[t] = nc;
             pa = na; pc = nc;
@@ -246,19 +312,20 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
         }
     }
     // reduced row 2c+1 (x_e): pa x_s + x_e + pc x_{e+1} = pd  (already normalised)
-    C2<T> A1 = pa, C1 = pc, D1[NR];
+    OT A1 = pa, C1 = pc;
+    C2<T> D1[NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) D1[r] = pd[r];
     // 2. upward sweep over rows e-2..s+1 (row e-1 keeps its downward form):
     //    row s+t becomes ab[t] x_s + x_{s+t} + cb[t] x_e = db[t]
 #pragma unroll
     for (int t = MMAX - 3; t >= 1; t--) {
-        const C2<T> cbt = cb[t];
-        const C2<T> na = cfms<T>(cbt, ab[t + 1], ab[t]);
-        const C2<T> nc = cneg<T>(cmul<T>(cbt, cb[t + 1]));
+        const OT cbt = cb[t];
+        const OT na = O::fms(cbt, ab[t + 1], ab[t]);
+        const OT nc = O::neg(O::mul(cbt, cb[t + 1]));
         C2<T> nd[NR];
 #pragma unroll
-        for (int r = 0; r < NR; r++) nd[r] = cfms<T>(cbt, db[r][t + 1], db[r][t]);
+        for (int r = 0; r < NR; r++) nd[r] = O::xfms(cbt, db[r][t + 1], db[r][t]);
         if (t <= len - 3) {
             ab[t] = na; cb[t] = nc;
 #pragma unroll
@@ -266,27 +333,28 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
         }
     }
     // reduced row 2c (x_s): sub_s x_{s-1} + B x_s + C x_e = D
-    C2<T> B0, C0, D0[NR];
+    OT B0, C0;
+    C2<T> D0[NR];
     if (len >= 3) {
-        B0 = cfms<T>(sup_s, ab[1], dia_s);
-        C0 = cneg<T>(cmul<T>(sup_s, cb[1]));
+        B0 = O::fms(sup_s, ab[1], dia_s);
+        C0 = O::neg(O::mul(sup_s, cb[1]));
 #pragma unroll
-        for (int r = 0; r < NR; r++) D0[r] = cfms<T>(sup_s, db[r][1], d_s[r]);
+        for (int r = 0; r < NR; r++) D0[r] = O::xfms(sup_s, db[r][1], d_s[r]);
     } else {
         B0 = dia_s; C0 = sup_s;
 #pragma unroll
         for (int r = 0; r < NR; r++) D0[r] = d_s[r];
     }
-    C2<T> A0;
+    OT A0;
     {
-        const C2<T> rc = crcp<T>(B0);
-        A0 = cmul<T>(sub_s, rc);
-        C0 = cmul<T>(C0, rc);
+        const OT rc = O::rcp(B0);
+        A0 = O::mul(sub_s, rc);
+        C0 = O::mul(C0, rc);
 #pragma unroll
-        for (int r = 0; r < NR; r++) D0[r] = cmul<T>(D0[r], rc);
+        for (int r = 0; r < NR; r++) D0[r] = O::xmul(rc, D0[r]);
     }
     if (!valid) {  // identity rows for idle threads and padding modes
-        A0 = C0 = A1 = C1 = mk<T>(0, 0);
+        A0 = C0 = A1 = C1 = O::zero();
 #pragma unroll
         for (int r = 0; r < NR; r++) D0[r] = D1[r] = mk<T>(0, 0);
     }
@@ -298,63 +366,65 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
     // two-way bank conflicts of this layout, measured 7% slower: 872 against
     // 811 us at N=1023 x 64 right-hand sides.)
     auto slot = [&](int q) { return q * W + w; };
-    C2<T> PA[2] = {A0, A1}, PC[2] = {C0, C1}, PD[2][NR];
+    OT PA[2] = {A0, A1}, PC[2] = {C0, C1};
+    C2<T> PD[2][NR];
 #pragma unroll
     for (int r = 0; r < NR; r++) { PD[0][r] = D0[r]; PD[1][r] = D1[r]; }
     if (c < P) {
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int q = 2 * c + h;
-            lds[slot(q)] = PA[h];
-            lds[RW + slot(q)] = PC[h];
+            bufA(0)[slot(q)] = PA[h];
+            bufC(0)[slot(q)] = PC[h];
 #pragma unroll
-            for (int r = 0; r < NR; r++) lds[(2 + r) * RW + slot(q)] = PD[h][r];
+            for (int r = 0; r < NR; r++) bufD(0)[r * RW + slot(q)] = PD[h][r];
         }
     }
     int cur = 0;
     for (int dd = 1; dd < RR; dd <<= 1) {
         __syncthreads();
-        const C2<T> *sb = lds + cur * BUF;
-        C2<T> *tb = lds + (cur ^ 1) * BUF;
+        const OT *sA = bufA(cur), *sC = bufC(cur);
+        const C2<T> *sD = bufD(cur);
+        OT *tA = bufA(cur ^ 1), *tC = bufC(cur ^ 1);
+        C2<T> *tD = bufD(cur ^ 1);
         if (c < P) {
             // row q: A x_{q-d} + x_q + C x_{q+d} = D, eliminated against rows q-d and q+d
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int q = 2 * c + h;
                 // neighbours outside the system read the row itself and are zeroed
-                const C2<T> z = mk<T>(0, 0);
                 const int qL = q - dd, qU = q + dd;
                 const bool hL = qL >= 0, hU = qU < RR;
                 const int iL = slot(hL ? qL : q), iU = slot(hU ? qU : q);
-                C2<T> aL = sb[iL], cL = sb[RW + iL], aU = sb[iU], cU = sb[RW + iU], dL[NR], dU[NR];
+                OT aL = sA[iL], cL = sC[iL], aU = sA[iU], cU = sC[iU];
+                C2<T> dL[NR], dU[NR];
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
-                    dL[r] = sb[(2 + r) * RW + iL];
-                    dU[r] = sb[(2 + r) * RW + iU];
+                    dL[r] = sD[r * RW + iL];
+                    dU[r] = sD[r * RW + iU];
                 }
                 if (!hL) {
-                    aL = z; cL = z;
+                    aL = cL = O::zero();
 #pragma unroll
-                    for (int r = 0; r < NR; r++) dL[r] = z;
+                    for (int r = 0; r < NR; r++) dL[r] = mk<T>(0, 0);
                 }
                 if (!hU) {
-                    aU = z; cU = z;
+                    aU = cU = O::zero();
 #pragma unroll
-                    for (int r = 0; r < NR; r++) dU[r] = z;
+                    for (int r = 0; r < NR; r++) dU[r] = mk<T>(0, 0);
                 }
-                const C2<T> A = PA[h], C = PC[h];
-                const C2<T> bb = csub<T>(mk<T>(1, 0), cadd<T>(cmul<T>(A, cL), cmul<T>(C, aU)));
-                const C2<T> ib = crcp<T>(bb);
+                const OT A = PA[h], C = PC[h];
+                const OT bb = O::fms2(A, cL, C, aU, O::one());
+                const OT ib = O::rcp(bb);
 #pragma unroll
                 for (int r = 0; r < NR; r++) {
-                    const C2<T> nd = csub<T>(PD[h][r], cadd<T>(cmul<T>(A, dL[r]), cmul<T>(C, dU[r])));
-                    PD[h][r] = cmul<T>(nd, ib);
-                    tb[(2 + r) * RW + slot(q)] = PD[h][r];
+                    PD[h][r] = O::xmul(ib, O::xfms2(A, dL[r], C, dU[r], PD[h][r]));
+                    tD[r * RW + slot(q)] = PD[h][r];
                 }
-                PA[h] = cmul<T>(cneg<T>(cmul<T>(A, aL)), ib);
-                PC[h] = cmul<T>(cneg<T>(cmul<T>(C, cU)), ib);
-                tb[slot(q)] = PA[h];
-                tb[RW + slot(q)] = PC[h];
+                PA[h] = O::neg(O::mul(O::mul(A, aL), ib));
+                PC[h] = O::neg(O::mul(O::mul(C, cU), ib));
+                tA[slot(q)] = PA[h];
+                tC[slot(q)] = PC[h];
             }
         }
         cur ^= 1;
@@ -372,7 +442,7 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
 #pragma unroll
         for (int t = 1; t < MMAX - 1; t++) {
             if (t < len - 1) {
-                C2<T> x = csub<T>(db[r][t], cadd<T>(cmul<T>(ab[t], xs), cmul<T>(cb[t], xe)));
+                const C2<T> x = O::xfms2(ab[t], xs, cb[t], xe, db[r][t]);
                 prob.store(b, md, s + t, x);
             }
         }
@@ -380,8 +450,11 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
     }
 }
 
-// LDS bytes of the reduced system: two buffers of A, C and NR right-hand sides.
-inline size_t tri_lds_bytes(size_t elem, int P, int W, int nr) { return elem * 2 * (2 + nr) * 2 * (size_t)P * W; }
+// LDS bytes of the reduced system: two buffers of A, C (operator values of
+// op bytes) and NR complex right-hand sides (cx bytes).
+inline size_t tri_lds_bytes(size_t op, size_t cx, int P, int W, int nr) {
+    return 2 * (2 * op + nr * cx) * 2 * (size_t)P * W;
+}
 
 template <typename T, int MM, int NR, class Prob>
 static hipError_t launch_tri_nr(const Prob &pb, int n, long nmodes, int batch, int nop, int W, int P,
@@ -390,7 +463,7 @@ static hipError_t launch_tri_nr(const Prob &pb, int n, long nmodes, int batch, i
     // every chunk needs at least two rows, at most MM rows
     if (P < 1 || P > n / 2 || (n + P - 1) / P > MM || W * P > 1024) return hipErrorInvalidValue;
     int threads = ((W * P + 63) / 64) * 64;
-    const size_t shm = tri_lds_bytes(sizeof(C2<T>), P, W, NR);
+    const size_t shm = tri_lds_bytes(sizeof(typename Prob::OT), sizeof(C2<T>), P, W, NR);
     if (shm > 160 * 1024) return hipErrorInvalidValue;
     // 512-thread bound: at most 256 VGPRs per lane and two waves per SIMD.
     // (16 FP64 rows spill a few registers there; measured faster than the
@@ -441,13 +514,23 @@ void tri_shape(int n, long nmodes, int prec, int *W, int *P, int *mmax) {
     *W = w; *P = p; *mmax = mm;
 }
 
+template <typename T, typename Ti, typename To, int MM, int DIM, bool REAL>
+static hipError_t launch_tri_r(const TriHelm &t, const void *in, void *out, hipStream_t st);
+
 template <typename T, typename Ti, typename To, int MM, int DIM>
 static hipError_t launch_tri_d(const TriHelm &t, const void *in, void *out, hipStream_t st) {
-    HelmProb<T, Ti, To, DIM> pb;
+    if (t.real) return launch_tri_r<T, Ti, To, MM, DIM, true>(t, in, out, st);
+    return launch_tri_r<T, Ti, To, MM, DIM, false>(t, in, out, st);
+}
+
+template <typename T, typename Ti, typename To, int MM, int DIM, bool REAL>
+static hipError_t launch_tri_r(const TriHelm &t, const void *in, void *out, hipStream_t st) {
+    HelmProb<T, Ti, To, DIM, REAL> pb;
     pb.bands.Nx = t.Nx;
-    pb.bands.coef = (const C2<T> *)t.coef;
+    // the real table holds the coefficients' real parts in pairs: half the entries
+    pb.bands.coef = (const C2<T> *)(REAL ? t.coefr : t.coef);
     pb.bands.nop = t.nop > 0 ? t.nop : 1;
-    pb.bands.cstride = t.cstride;
+    pb.bands.cstride = REAL ? t.cstride / 2 : t.cstride;
     pb.bands.lamx = t.lamx;
     pb.bands.lamy = t.lamy;
     pb.in = (const C2<Ti> *)in;

@@ -257,6 +257,15 @@ class Plan:
         _check(lib().hh_plan_last_refine(self.h, ctypes.byref(s), ctypes.byref(r)))
         return s.value, r.value
 
+    def set_real_operator(self, mode):
+        """-1: real-band mode solves whenever every coefficient is real (default);
+        0: complex-band mode solves always."""
+        _check(lib().hh_plan_set_real_operator(self.h, int(mode)))
+
+    @property
+    def is_real_operator(self):
+        return bool(lib().hh_plan_is_real_operator(self.h))
+
     def set_operators(self, coef):
         """Replace the operators (at most as many as the plan was created with)."""
         nvec, n = (6, self.Ny) if self.dim == 2 else (12, self.Nz)

@@ -141,7 +141,10 @@ int hh_plan_set_operators(hh_plan *plan, int nop, const double *coef);
  * thread (the operator's pivots are computed once for them).  Set by the
  * autotuner; (0, 0, *) restores the built-in heuristic.  Shapes differ in
  * speed only: every right-hand side gives the same bits for a given W and
- * rows_per_thread, whatever else is in its batch and whatever rhs_per_thread. */
+ * rows_per_thread, whatever else is in its batch and whatever rhs_per_thread.
+ * get_tri_shape reports the shape a solve uses; for the built-in heuristic
+ * it reports rhs_per_thread = 1, though a solve with real bands in FP64 at
+ * 16 rows per thread takes 2 when its batch gives 2048 blocks or more. */
 int hh_plan_set_tri_shape(hh_plan *plan, int W, int rows_per_thread, int rhs_per_thread);
 int hh_plan_get_tri_shape(const hh_plan *plan, int *W, int *P, int *rows_per_thread, int *rhs_per_thread);
 /* 2D FFT path: layout of the spectrum between the row DSTs and the mode
@@ -151,6 +154,17 @@ int hh_plan_get_tri_shape(const hh_plan *plan, int *W, int *P, int *rows_per_thr
  * tiles of the tridiagonal W.  Default 0.  Tiles make the mode solves
  * faster and the row DSTs slower; layouts differ in speed only. */
 int hh_plan_set_tile_width(hh_plan *plan, int tw);
+/* Mode solves with real bands.  When every stencil coefficient of the
+ * plan's operators has a zero imaginary part (real k), the tridiagonal
+ * kernels run with real band arithmetic (complex right-hand sides); this
+ * is chosen again by every hh_plan_set_operators.  mode -1 = automatic
+ * (default), 0 = complex-band kernels always.  Both give the oracle's
+ * solution to the same tolerance; they differ in rounding and speed.
+ * No counterpart in the reference (its LU factors are complex,
+ * 2D_Program/main_functions/LU.c:3-46).  is_real_operator: 1 if the next
+ * solve uses the real-band kernels. */
+int hh_plan_set_real_operator(hh_plan *plan, int mode);
+int hh_plan_is_real_operator(const hh_plan *plan);
 
 /* ---- graphs: a fixed list of solves replayed with one launch ----
  * No reference counterpart (the reference solves one problem per call): for

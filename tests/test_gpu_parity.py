@@ -140,6 +140,32 @@ def test_solve2d_rectangular(H, ctx, oracle):
     assert relerr(host(plan.solve(dev(s["rhs"]))), s["sol"]) <= 1e-12
 
 
+@pytest.mark.parametrize("dim,order,kind,p,N,real", [
+    (2, 6, 1, [50.0, 0.0, 7, 11, 1.3], 1023, True), (2, 4, 1, [10.0, 0.0, 3, 2, 0.0], 255, True),
+    (2, 6, 1, [80.0, 0.5, 5, 9, 0.2], 127, False), (2, 6, 0, None, 100, False),
+    (3, 6, 1, [30.0, 0.0, 3, 2, 1, 1.0, 1.0], 63, True), (3, 4, 1, [30.0, 0.0, 5, 1, 2, 1.0, 1.0], 31, True)])
+def test_real_and_complex_band_kernels(H, ctx, oracle, dim, order, kind, p, N, real):
+    """Real k makes every band real and the mode solves use real-band
+    arithmetic; both kernels meet the 1e-12 bar against the oracle and agree
+    with each other to rounding."""
+    say(f"band kernels dim={dim} order={order} kind={kind} N={N} real={real}")
+    s = oracle.solve2d(order, N, kind, p, threads=8) if dim == 2 else oracle.solve3d(order, N, kind, p, threads=8)
+    plan = H.Plan2D(ctx, N, N, s["coef"]) if dim == 2 else H.Plan3D(ctx, N, N, N, s["coef"])
+    assert plan.is_real_operator == real
+    auto = host(plan.solve(dev(s["rhs"])))
+    plan.set_real_operator(0)
+    assert not plan.is_real_operator
+    cplx = host(plan.solve(dev(s["rhs"])))
+    ea, ec = relerr(auto, s["sol"]), relerr(cplx, s["sol"])
+    say(f"  rel err vs oracle: automatic {ea:.2e}, complex bands {ec:.2e}")
+    assert ea <= 1e-12 and ec <= 1e-12
+    assert relerr(auto, cplx) <= 1e-12
+    plan.set_real_operator(-1)
+    assert plan.is_real_operator == real
+    with pytest.raises(H.HHError):
+        plan.set_real_operator(1)
+
+
 @pytest.mark.parametrize("N", [63, 1023])
 def test_batch_alone_equals_in_batch(H, ctx, oracle, N):
     """Each right-hand side gives the same bits alone and inside a batch."""
