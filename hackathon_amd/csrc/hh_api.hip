@@ -244,20 +244,32 @@ static int get_half_table(hh_context *c, int N, int f32, void **out) {
 }
 
 // DST variant for lines of length N.  Automatic choice: the half-length
-// transform (measured 6% faster than the odd extension at N+1 = 1024) up to
-// N+1 = 1024; above, its odd outputs, prefix sums of M/2 transform values,
-// carry roundoff growing like sqrt(M) (2e-13 relative at N+1 = 4096), so the
-// odd-extension transform (error growing like log M) is used.
+// transform at every power-of-two N+1: 20-45% faster than the odd extension
+// above N+1 = 1024 (503 against 636 us for 1 GB of rows at N+1 = 2048, 566
+// against 1055 at 4096), and accurate to 7e-15 relative per transform at
+// N+1 = 4096; its error is correlated across the spectrum (prefix sums of
+// transform values), so a transform and its inverse give 7e-14 there
+// against 5e-16 for the odd extension, far inside a solve's 1e-12.
 static int dst_algo_for(const hh_context *c, int N) {
     if (c->dst_algo >= 0) return c->dst_algo;
-    return N + 1 > 1024 ? 1 : 0;
+    return 0;
+}
+
+// Block size of the automatic choice: one-wave blocks (one line each) for
+// contiguous rows of N+1 >= 512, measured 3-8% faster there (N+1 = 512: 446
+// against 484 us per GB of rows; 1024: 496 against 521) and slower at 256;
+// 256-thread blocks for strided lines, whose tiles they widen.
+static int dst_algo_lines(const hh_context *c, const hh::Lines &ln) {
+    int algo = dst_algo_for(c, ln.N);
+    if (c->dst_algo < 0 && ln.elem_stride == 1 && !ln.in_tw && !ln.out_tw && ln.N + 1 >= 512) algo |= 2;
+    return algo;
 }
 
 // One DST pass over lines; in-place when the FFT path applies.
 static int dst_pass(hh_context *c, const hh::Lines &ln, int ti, int tc, int to, const void *in, void *out) {
     void *tw = nullptr, *S = nullptr;
     int rc;
-    const int algo = dst_algo_for(c, ln.N);
+    const int algo = dst_algo_lines(c, ln);
     if (hh::fft_supported(ln.N))
         rc = (algo & 1) == 0 ? get_half_table(c, ln.N, tc, &tw) : get_twiddles(c, ln.N, tc, &tw);
     else rc = get_dst_matrix(c, ln.N, tc, &S);

@@ -62,8 +62,10 @@ int hh_context_enable_timing(hh_context *ctx, int on);
  * symmetrised row plus a prefix-sum post-pass, 1 = 2(N+1)-point FFT of the
  * odd extension; bit 1: 64-thread blocks instead of 256.  All four are
  * orthonormal DST-I to roundoff; the prefix sums of the first make its error
- * grow like sqrt(N) rather than log N.  -1 (default): the (N+1)-point
- * transform up to N+1 = 1024, the odd extension above. */
+ * grow like sqrt(N) rather than log N (7e-14 relative for a transform and
+ * its inverse at N+1 = 4096, against 5e-16).  -1 (default): the
+ * (N+1)-point transform, in 64-thread blocks for contiguous rows with
+ * N+1 >= 512 and 256-thread blocks otherwise. */
 int hh_context_set_dst_algorithm(hh_context *ctx, int algo);
 int hh_context_read_timing(hh_context *ctx, double ms[5], long launches[5], double bytes[5]);
 int hh_context_reset_timing(hh_context *ctx);

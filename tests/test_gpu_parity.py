@@ -46,9 +46,10 @@ def relerr(a, b):
 
 
 # ------------------------------------------------------------------ DST
-@pytest.fixture(params=[0, 1, 2, 3], ids=["half", "oddext", "half64", "oddext64"])
+@pytest.fixture(params=[-1, 0, 1, 2, 3], ids=["auto", "half", "oddext", "half64", "oddext64"])
 def dctx(ctx, request):
-    """The context with each DST variant: (N+1)-point or odd-extension FFT, 256- or 64-thread blocks."""
+    """The context with each DST variant: (N+1)-point or odd-extension FFT, 256- or 64-thread blocks,
+    and the automatic choice ((N+1)-point, block size by line length)."""
     ctx.set_dst_algorithm(request.param)
     yield ctx
     ctx.set_dst_algorithm(-1)
@@ -70,6 +71,8 @@ def dst_tol(algo, N):
     roundoff grows like log N; the half-length transform's odd outputs are
     prefix sums of N/2 transform values, so its roundoff grows like sqrt(N)
     (measured 1.0e-13 after two transforms at N = 4095)."""
+    if algo < 0:
+        algo = 0   # the automatic choice is the half-length transform
     return 1e-13 if (algo & 1) or N < 1023 else 1e-13 * np.sqrt((N + 1) / 1024.0) * 2
 
 
