@@ -312,20 +312,21 @@ def run_uneven(args, torch, H):
             worst = max(worst, d / mx)
         out[name]["max_disc_err_vs_exact"] = worst
         g.close()
-    check = {}
+    check = {"fp64": {}, "mixed": {}}
     if not args.no_check:
         from oracle import oracle as O
         log("uneven: oracle checks")
-        for (pmx, F, sol), gi in zip(solvesmx, info):
+        for (p64, F, sol64), (pmx, _, solmx), gi in zip(solves64, solvesmx, info):
             if gi["N"] > 1023:
                 continue
             N = gi["N"]
             LUU = O.lu2d(N, N, gi["ops"][0], 16)
             ref = O.solver2d(N, N, F[0].cpu().numpy().ravel(), LUU, 16)
-            g = sol[0].cpu().numpy().ravel()
-            e = float(np.abs(g - ref).max() / np.abs(ref).max())
-            check[f"N{N}_order{gi['order']}"] = e
-        out["mixed"]["max_rel_err_vs_oracle"] = max(check.values()) if check else None
+            for name, sol in (("fp64", sol64), ("mixed", solmx)):
+                g = sol[0].cpu().numpy().ravel()
+                check[name][f"N{N}_order{gi['order']}"] = float(np.abs(g - ref).max() / np.abs(ref).max())
+        for name in ("fp64", "mixed"):
+            out[name]["max_rel_err_vs_oracle"] = max(check[name].values()) if check[name] else None
     ok = out["mixed"].get("max_rel_err_vs_oracle") is not None and out["mixed"]["max_rel_err_vs_oracle"] <= 1e-10
     best = "mixed" if ok and out["mixed"]["dof_per_s"] >= out["fp64"]["dof_per_s"] else "fp64"
     sizes = {str(k): len(v) for k, v in sorted(groups.items())}
