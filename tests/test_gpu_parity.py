@@ -116,6 +116,7 @@ CASES2D = [
     (4, 1, [10.0, 0.0, 3, 2, 0.0], 255), (6, 1, [10.0, 0.0, 3, 2, 0.0], 100),
     (6, 1, [50.0, 0.0, 7, 11, 1.3], 1023), (4, 1, [50.0, 0.0, 20, 1, 4.0], 1023),
     (6, 1, [80.0, 0.5, 5, 9, 0.2], 2047), (4, 1, [30.0, 0.0, 2, 3, 0.0], 2),
+    (6, 1, [60.0, 0.0, 9, 4, 0.7], 2047), (4, 1, [25.0, 0.0, 3, 8, 2.1], 511),
 ]
 
 
@@ -268,7 +269,7 @@ def test_rhs3d(H, ctx, oracle, order, kind, p, N):
 
 
 # ------------------------------------------------------------------ mixed precision
-@pytest.mark.parametrize("dim,order,N", [(2, 6, 255), (2, 4, 1023), (3, 6, 63)])
+@pytest.mark.parametrize("dim,order,N", [(2, 6, 255), (2, 4, 1023), (2, 6, 2047), (3, 6, 63)])
 def test_mixed_precision(H, ctx, oracle, dim, order, N):
     say(f"mixed dim={dim} order={order} N={N}")
     if dim == 2:
