@@ -132,7 +132,9 @@ int request_coef(const hh_request &r, std::vector<double> &out) {
 // coefficient tables and eigenvalues of max_batch operators, and the
 // solve's workspace (estimated; refined once the plan exists).
 size_t resident_bytes(const Shape &s, int cap, int max_batch) {
-    const size_t tables = (size_t)max_batch * (s.dim == 2 ? 6 : 12) * s.sys() * 24 + 8 * (s.Nx + s.Ny);
+    // coefficient tables per entry: complex FP64 and FP32 (16 + 8 B) and
+    // their real parts (8 + 4 B) for the real-band kernels
+    const size_t tables = (size_t)max_batch * (s.dim == 2 ? 6 : 12) * s.sys() * 36 + 8 * (s.Nx + s.Ny);
     // context scratch of a solve: the 2D mode-tiled spectrum (16 B per DOF in
     // FP64, 8 B in FP32 plus the 8 B FP32 correction), the 3D FP32 correction
     const size_t ws = (size_t)cap * s.field() * (s.dim == 2 ? 16 : (s.prec == HH_PREC_MIXED ? 8 : 0));
@@ -285,9 +287,12 @@ void lane_main(hh_server *S, int lane) {
                 q->done = true;
             }
             R.last_use = ++S->clock;
+            // a failed batch leaves no half-built grid behind: its bytes go
+            // back to the budget (eviction only finds grids with a plan)
+            if (rc != HH_OK) release(S, R);
             S->busy[lane] = 0;
             S->batches++;
-            S->solved += nb;
+            if (rc == HH_OK) S->solved += nb;
         }
         S->cv_done.notify_all();
         S->cv_work.notify_all();
