@@ -324,50 +324,100 @@ __global__ __launch_bounds__(256) void k_residual2d(int Nx, int Ny, int nchunk, 
     }
 }
 
-// 3 x 3 neighbourhood of (i, j) in layer ll of one right-hand side
-struct Nb3 { z2 v[3][3]; };   // v[dj+1][di+1]
+
+// Tile of the 3D residual: RX x RY points of a layer per block, marching
+// RZ layers along z.  Each layer's tile with its one-point halo goes
+// through LDS once, loaded coalesced (three layer buffers in rotation); a
+// thread reads the new layer's nine values from LDS per step and keeps, for
+// each of its three layers, the four sums the stencil weights.  The loads are software-
+// pipelined: a step stores the layer fetched during the previous step and
+// issues the fetch of the next one (and of the next F value) before its
+// barrier, so HBM latency is not paid once per layer.
+constexpr int RX = 64, RY = 8, RZ = 16;
 
 template <typename To>
-__global__ __launch_bounds__(256) void k_residual3d(int Nx, int Ny, int Nz, int nchunk, const z2 *coef, int nop,
-                                                    const z2 *u, const z2 *F, C2<To> *res, int b0) {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
+__global__ __launch_bounds__(RX * RY) void k_residual3d(int Nx, int Ny, int Nz, int nchunk, const z2 *coef, int nop,
+                                                       const z2 *u, const z2 *F, C2<To> *res, int b0) {
+    constexpr int HX = RX + 2, HY = RY + 2, HE = HX * HY, NT = RX * RY;
+    static_assert(HE <= 2 * NT, "a layer tile with halo takes at most two loads per thread");
+    __shared__ z2 lay[3][HE];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * RX + tx;
+    const int i0 = blockIdx.x * RX, j0 = blockIdx.y * RY;
+    const int i = i0 + tx, j = j0 + ty;
     const int zc = (int)blockIdx.z % nchunk;
     const int b = b0 + (int)blockIdx.z / nchunk;
-    if (i >= Nx || j >= Ny) return;
     const long Nxy = (long)Nx * Ny, N3 = Nxy * Nz;
     const z2 *ub = u + b * N3;
-    auto plane = [&](int ll, Nb3 &p) {
-#pragma unroll
-        for (int dj = -1; dj <= 1; dj++)
-#pragma unroll
-            for (int di = -1; di <= 1; di++) {
-                const int ii = i + di, jj = j + dj;
-                const bool in = ll >= 0 && ll < Nz && ii >= 0 && ii < Nx && jj >= 0 && jj < Ny;
-                p.v[dj + 1][di + 1] = in ? ub[ii + (long)jj * Nx + (long)ll * Nxy] : zr(0.0);
-            }
+    // the (at most two) halo-tile elements this thread loads; points outside
+    // the grid read zero (the reference's zero ghosts)
+    const int e1 = tid + NT;
+    const bool has1 = e1 < HE;
+    auto src = [&](int e, int ll, bool &in) -> long {
+        const int hy = e / HX, hx = e - hy * HX;
+        const int ii = i0 + hx - 1, jj = j0 + hy - 1;
+        in = ll >= 0 && ll < Nz && ii >= 0 && ii < Nx && jj >= 0 && jj < Ny;
+        return in ? ii + (long)jj * Nx + (long)ll * Nxy : 0;
+    };
+    z2 n0, n1;   // the next layer's elements, in flight during a step
+    auto fetch = [&](int ll) {
+        bool in0, in1;
+        const long s0 = src(tid, ll, in0), s1 = src(has1 ? e1 : tid, ll, in1);
+        const z2 v0 = ub[s0], v1 = ub[s1];
+        n0 = in0 ? v0 : zr(0.0);
+        n1 = in1 ? v1 : zr(0.0);
+    };
+    auto stash = [&](int slot) {
+        lay[slot][tid] = n0;
+        if (has1) lay[slot][e1] = n1;
+    };
+    // a layer's 3 x 3 neighbourhood enters only through four coefficient-free
+    // sums (corners, x and y neighbours, centre), kept instead of the nine values
+    struct Part { z2 corner, xn, yn, ctr; };
+    auto plane = [&](int slot, Part &p) {
+        const z2 *q = &lay[slot][(ty + 1) * HX + tx + 1];
+        p.corner = za(za(za(q[HX + 1], q[HX - 1]), q[-HX + 1]), q[-HX - 1]);
+        p.xn = za(q[1], q[-1]);
+        p.yn = za(q[HX], q[-HX]);
+        p.ctr = q[0];
     };
     // one layer's contribution, coefficients c = (a, b, c, d) of that layer set
-    auto layer = [&](const Nb3 &p, const z2 *c) -> z2 {
-        const z2 corner = za(za(za(p.v[2][2], p.v[2][0]), p.v[0][2]), p.v[0][0]);
-        const z2 xn = za(p.v[1][2], p.v[1][0]);
-        const z2 yn = za(p.v[2][1], p.v[0][1]);
-        return za(za(za(zm(c[3], p.v[1][1]), zm(c[0], corner)), zm(c[1], xn)), zm(c[2], yn));
+    auto layer = [&](const Part &p, const z2 *c) -> z2 {
+        return za(za(za(zm(c[3], p.ctr), zm(c[0], p.corner)), zm(c[1], p.xn)), zm(c[2], p.yn));
     };
-    const int l0 = zc * RES_ROWS, l1 = min(l0 + RES_ROWS, Nz);
-    Nb3 pm, pc, pp;
-    plane(l0 - 1, pm);
-    plane(l0, pc);
+    const int l0 = zc * RZ, l1 = min(l0 + RZ, Nz);
+    const bool out = i < Nx && j < Ny;
+    const long id0 = b * N3 + (long)(out ? j : 0) * Nx + (out ? i : 0);
+    fetch(l0 - 1);
+    stash(0);
+    fetch(l0);
+    stash(1);
+    fetch(l0 + 1);
+    z2 fn = F[id0 + (long)l0 * Nxy];
+    __syncthreads();
+    Part pm, pc, pp;
+    plane(0, pm);
+    plane(1, pc);
     for (int l = l0; l < l1; l++) {
-        plane(l + 1, pp);
-        const z2 *cl = coef + 12 * ((long)(b % nop) * Nz + l);   // a..d, ap..dp, am..dm
-        z2 acc = layer(pm, cl + 8);
-        acc = za(acc, layer(pc, cl));
-        acc = za(acc, layer(pp, cl + 4));
-        const long id = b * N3 + (long)l * Nxy + (long)j * Nx + i;
-        res[id] = cconv<To, double>(zd(acc, F[id]));
+        const int sn = (l - l0 + 2) % 3;   // the slot of layer l-2, last read a barrier ago
+        stash(sn);                         // layer l+1
+        const z2 f = fn;
+        if (l + 1 < l1) {
+            fetch(l + 2);
+            fn = F[id0 + (long)(l + 1) * Nxy];
+        }
+        __syncthreads();
+        plane(sn, pp);
+        if (out) {
+            const z2 *cl = coef + 12 * ((long)(b % nop) * Nz + l);   // a..d, ap..dp, am..dm
+            z2 acc = layer(pm, cl + 8);
+            acc = za(acc, layer(pc, cl));
+            acc = za(acc, layer(pp, cl + 4));
+            res[id0 + (long)l * Nxy] = cconv<To, double>(zd(acc, f));
+        }
         pm = pc;
         pc = pp;
+        // slot sn is next rewritten two steps on; every thread has passed
+        // the next step's barrier, after its read here, by then
     }
 }
 
@@ -389,17 +439,17 @@ hipError_t launch_residual2d(int Nx, int Ny, int batch, const void *coef, int no
 
 hipError_t launch_residual3d(int Nx, int Ny, int Nz, int batch, const void *coef, int nop, const void *u,
                              const void *F, void *res, int res_f32, hipStream_t st) {
-    const int nchunk = (Nz + RES_ROWS - 1) / RES_ROWS;
+    const int nchunk = (Nz + RZ - 1) / RZ;
     const int per = 65535 / nchunk;   // right-hand sides per launch (grid z)
     if (per < 1) return hipErrorInvalidValue;
     for (int b0 = 0; b0 < batch; b0 += per) {
         const int nb = batch - b0 < per ? batch - b0 : per;
-        dim3 g((unsigned)((Nx + 63) / 64), (unsigned)((Ny + 3) / 4), (unsigned)(nchunk * nb));
+        dim3 g((unsigned)((Nx + RX - 1) / RX), (unsigned)((Ny + RY - 1) / RY), (unsigned)(nchunk * nb));
         if (res_f32)
-            hipLaunchKernelGGL(k_residual3d<float>, g, dim3(64, 4), 0, st, Nx, Ny, Nz, nchunk, (const z2 *)coef, nop,
-                               (const z2 *)u, (const z2 *)F, (float2 *)res, b0);
+            hipLaunchKernelGGL(k_residual3d<float>, g, dim3(RX, RY), 0, st, Nx, Ny, Nz, nchunk, (const z2 *)coef,
+                               nop, (const z2 *)u, (const z2 *)F, (float2 *)res, b0);
         else
-            hipLaunchKernelGGL(k_residual3d<double>, g, dim3(64, 4), 0, st, Nx, Ny, Nz, nchunk, (const z2 *)coef,
+            hipLaunchKernelGGL(k_residual3d<double>, g, dim3(RX, RY), 0, st, Nx, Ny, Nz, nchunk, (const z2 *)coef,
                                nop, (const z2 *)u, (const z2 *)F, (z2 *)res, b0);
     }
     return hipGetLastError();
