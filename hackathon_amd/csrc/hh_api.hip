@@ -327,7 +327,7 @@ extern "C" int hh_context_create(int device, hh_context **out) {
     e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail(HH_ERR_DEVICE, "stream creation failed: %s", hipGetErrorString(e)); }
     c->stream = c->own;
-    if (hipMalloc((void **)&c->d_norm, 2 * sizeof(double)) != hipSuccess) {
+    if (hipMalloc((void **)&c->d_norm, hh::norm_buffer_doubles() * sizeof(double)) != hipSuccess) {
         (void)hipStreamDestroy(c->own);
         delete c;
         return fail(HH_ERR_NOMEM, "norm buffer allocation failed");

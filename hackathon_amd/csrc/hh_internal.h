@@ -119,7 +119,10 @@ hipError_t launch_residual3d(int Nx, int Ny, int Nz, int batch, const void *coef
                              const void *F, void *res, int res_f32, hipStream_t st);
 // u += s*c (c in FP32 or FP64)
 hipError_t launch_axpy_update(long n, void *u, const void *c, int c_f32, double s, hipStream_t st);
-// out[0] = max |x|, out[1] = max |x - y| (y may be null); out is device memory
+// out[0] = max |x|, out[1] = max |x - y| (y may be null); out is device
+// memory of norm_buffer_doubles() values, out[2..] the per-block partials
+constexpr int NORM_BLOCKS = 2048;
+constexpr long norm_buffer_doubles() { return 2 + 2L * NORM_BLOCKS; }
 hipError_t launch_norms(long n, const void *x, int x_f32, const void *y, double *out, hipStream_t st);
 hipError_t launch_convert(long n, const void *in, int in_f32, void *out, int out_f32, hipStream_t st);
 

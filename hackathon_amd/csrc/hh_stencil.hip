@@ -484,17 +484,36 @@ hipError_t launch_convert(long n, const void *in, int in_f32, void *out, int out
     return hipGetLastError();
 }
 
-// max |x| and max |x - y| over n complex values; the maxima of non-negative
-// doubles are combined through their bit patterns (monotone as unsigned
-// integers), so the result does not depend on the order of the atomics.
+// max |x| and max |x - y| over n complex values, in two passes without
+// atomics: each block writes the maxima of |x|^2 and |x - y|^2 over its
+// share to part[blockIdx.x] and part[NORM_BLOCKS + blockIdx.x], then one
+// block combines them and roots the result once (sqrt is correctly rounded
+// and monotonic, so this equals the maximum of the rooted values).  Four
+// values per thread and step are loaded before any is used.
 template <typename Tx>
-__global__ void k_norms(long n, const C2<Tx> *x, const z2 *y, unsigned long long *out) {
+__global__ __launch_bounds__(256) void k_norms(long n, const C2<Tx> *x, const z2 *y, double *part) {
     __shared__ double s0[256], s1[256];
     double m0 = 0, m1 = 0;
-    for (long id = (long)blockIdx.x * blockDim.x + threadIdx.x; id < n; id += (long)gridDim.x * blockDim.x) {
-        z2 a = cconv<double, Tx>(x[id]);
-        m0 = fmax(m0, sqrt(cabs2<double>(a)));
-        if (y) m1 = fmax(m1, sqrt(cabs2<double>(zd(a, y[id]))));
+    const long stride = (long)gridDim.x * blockDim.x;
+    long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; id + 3 * stride < n; id += 4 * stride) {
+        z2 a[4], c[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) a[u] = cconv<double, Tx>(x[id + u * stride]);
+        if (y) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) c[u] = y[id + u * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            m0 = fmax(m0, cabs2<double>(a[u]));
+            if (y) m1 = fmax(m1, cabs2<double>(zd(a[u], c[u])));
+        }
+    }
+    for (; id < n; id += stride) {
+        const z2 a = cconv<double, Tx>(x[id]);
+        m0 = fmax(m0, cabs2<double>(a));
+        if (y) m1 = fmax(m1, cabs2<double>(zd(a, y[id])));
     }
     s0[threadIdx.x] = m0;
     s1[threadIdx.x] = m1;
@@ -507,26 +526,46 @@ __global__ void k_norms(long n, const C2<Tx> *x, const z2 *y, unsigned long long
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        atomicMax(out, (unsigned long long)__double_as_longlong(s0[0]));
-        atomicMax(out + 1, (unsigned long long)__double_as_longlong(s1[0]));
+        part[blockIdx.x] = s0[0];
+        part[NORM_BLOCKS + blockIdx.x] = s1[0];
     }
 }
 
-__global__ void k_zero2(unsigned long long *p) {
-    if (threadIdx.x < 2) p[threadIdx.x] = 0ull;
+__global__ __launch_bounds__(256) void k_norms_final(int nb, const double *part, double *out) {
+    __shared__ double s0[256], s1[256];
+    double m0 = 0, m1 = 0;
+    for (int b = threadIdx.x; b < nb; b += 256) {
+        m0 = fmax(m0, part[b]);
+        m1 = fmax(m1, part[NORM_BLOCKS + b]);
+    }
+    s0[threadIdx.x] = m0;
+    s1[threadIdx.x] = m1;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            s0[threadIdx.x] = fmax(s0[threadIdx.x], s0[threadIdx.x + s]);
+            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sqrt(s0[0]);
+        out[1] = sqrt(s1[0]);
+    }
 }
 
 hipError_t launch_norms(long n, const void *x, int x_f32, const void *y, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_zero2, dim3(1), dim3(64), 0, st, (unsigned long long *)out);
-    long blocks = (n + 255) / 256;
-    if (blocks > 2048) blocks = 2048;
+    long blocks = (n + 2047) / 2048;   // about eight values per thread
+    if (blocks > NORM_BLOCKS) blocks = NORM_BLOCKS;
     if (blocks < 1) blocks = 1;
+    double *part = out + 2;
     if (x_f32)
         hipLaunchKernelGGL(k_norms<float>, dim3((unsigned)blocks), dim3(256), 0, st, n, (const float2 *)x,
-                           (const z2 *)y, (unsigned long long *)out);
+                           (const z2 *)y, part);
     else
         hipLaunchKernelGGL(k_norms<double>, dim3((unsigned)blocks), dim3(256), 0, st, n, (const z2 *)x,
-                           (const z2 *)y, (unsigned long long *)out);
+                           (const z2 *)y, part);
+    hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(256), 0, st, (int)blocks, (const double *)part, out);
     return hipGetLastError();
 }
 
