@@ -258,10 +258,14 @@ static int dst_algo_for(const hh_context *c, int N) {
 // Block size of the automatic choice: one-wave blocks (one line each) for
 // contiguous rows of N+1 >= 512, measured 3-8% faster there (N+1 = 512: 446
 // against 484 us per GB of rows; 1024: 496 against 521) and slower at 256;
-// 256-thread blocks for strided lines, whose tiles they widen.
-static int dst_algo_lines(const hh_context *c, const hh::Lines &ln) {
+// 256-thread blocks for strided lines, whose tiles they widen; in FP32,
+// 512-thread blocks (tiles of 32 lines at N+1 = 256: 69 against 74 us per
+// 255^3 pass; in FP64 such a tile's LDS leaves one block per CU and it is
+// slower, 120 against 116 us).
+static int dst_algo_lines(const hh_context *c, const hh::Lines &ln, int tc) {
     int algo = dst_algo_for(c, ln.N);
     if (c->dst_algo < 0 && ln.elem_stride == 1 && !ln.in_tw && !ln.out_tw && ln.N + 1 >= 512) algo |= 2;
+    if (c->dst_algo < 0 && ln.elem_stride != 1 && tc && ln.N + 1 >= 128 && ln.N + 1 <= 256) algo |= 4;
     return algo;
 }
 
@@ -269,7 +273,7 @@ static int dst_algo_lines(const hh_context *c, const hh::Lines &ln) {
 static int dst_pass(hh_context *c, const hh::Lines &ln, int ti, int tc, int to, const void *in, void *out) {
     void *tw = nullptr, *S = nullptr;
     int rc;
-    const int algo = dst_algo_lines(c, ln);
+    const int algo = dst_algo_lines(c, ln, tc);
     if (hh::fft_supported(ln.N))
         rc = (algo & 1) == 0 ? get_half_table(c, ln.N, tc, &tw) : get_twiddles(c, ln.N, tc, &tw);
     else rc = get_dst_matrix(c, ln.N, tc, &S);
@@ -397,7 +401,7 @@ static int collect_timing(hh_context *c) {
 
 extern "C" int hh_context_set_dst_algorithm(hh_context *c, int algo) {
     if (!c) return fail(HH_ERR_ARG, "null context");
-    if (algo < -1 || algo > 3) return fail(HH_ERR_ARG, "DST algorithm must be -1..3 (got %d)", algo);
+    if (algo < -1 || algo > 7) return fail(HH_ERR_ARG, "DST algorithm must be -1..7 (got %d)", algo);
     std::lock_guard<std::mutex> lk(c->mu);
     c->dst_algo = algo;
     return HH_OK;

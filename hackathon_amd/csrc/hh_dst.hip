@@ -534,7 +534,8 @@ static hipError_t launch_fft_L(const Lines &ln, const void *in, void *out, const
 }
 
 template <typename Ti, typename T, typename To, int M, int TPB>
-static hipError_t launch_half_M(const Lines &ln, const void *in, void *out, const void *aux, hipStream_t st) {
+static hipError_t launch_half_M(const Lines &ln, const void *in, void *out, const void *aux, bool wide,
+                                hipStream_t st) {
     using cfg = half_cfg<M, TPB>;
     constexpr int NW = cfg::P > 64 ? cfg::P / 64 : 1;
     size_t shm = sizeof(T) * (cfg::LPB * cfg::LS + 2 * cfg::LPB * NW);
@@ -556,6 +557,24 @@ static hipError_t launch_half_M(const Lines &ln, const void *in, void *out, cons
         return hipGetLastError();
     }
     if (ln.col_stride != 1) return hipErrorInvalidValue;
+    if constexpr (TPB == 256 && M >= 128 && M <= 256) {
+        using wcfg = half_cfg<M, 512>;
+        constexpr int WNW = wcfg::P > 64 ? wcfg::P / 64 : 1;
+        constexpr size_t wshm = sizeof(T) * (wcfg::LPB * wcfg::LS + 2 * wcfg::LPB * WNW);
+        if (wide && wshm <= 160 * 1024) {   // tiles of 512/P lines: twice the row-segment width
+            auto kfn = k_dst_half_cols<Ti, T, To, M, 512>;
+            if (wshm > 65536) {
+                hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                   (int)wshm);
+                if (e != hipSuccess) return e;
+            }
+            long ntiles = (ln.ncols + wcfg::LPB - 1) / wcfg::LPB;
+            long grid = ntiles * ln.nplanes;
+            hipLaunchKernelGGL(kfn, dim3((unsigned)grid), dim3(wcfg::THREADS), wshm, st, pi, po, ln.ncols, ntiles,
+                               ln.plane_stride, ln.elem_stride, scale, pa);
+            return hipGetLastError();
+        }
+    }
     auto kfn = k_dst_half_cols<Ti, T, To, M, TPB>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
@@ -569,21 +588,22 @@ static hipError_t launch_half_M(const Lines &ln, const void *in, void *out, cons
 }
 
 template <typename Ti, typename T, typename To, int TPB>
-static hipError_t launch_fft_t(const Lines &ln, int odd, const void *in, void *out, const void *tw, hipStream_t st) {
-    if (!odd) {
+static hipError_t launch_fft_t(const Lines &ln, int algo, const void *in, void *out, const void *tw, hipStream_t st) {
+    const bool wide = (algo & 4) != 0;
+    if (!(algo & 1)) {
         switch (ln.N + 1) {
-        case 2: return launch_half_M<Ti, T, To, 2, TPB>(ln, in, out, tw, st);
-        case 4: return launch_half_M<Ti, T, To, 4, TPB>(ln, in, out, tw, st);
-        case 8: return launch_half_M<Ti, T, To, 8, TPB>(ln, in, out, tw, st);
-        case 16: return launch_half_M<Ti, T, To, 16, TPB>(ln, in, out, tw, st);
-        case 32: return launch_half_M<Ti, T, To, 32, TPB>(ln, in, out, tw, st);
-        case 64: return launch_half_M<Ti, T, To, 64, TPB>(ln, in, out, tw, st);
-        case 128: return launch_half_M<Ti, T, To, 128, TPB>(ln, in, out, tw, st);
-        case 256: return launch_half_M<Ti, T, To, 256, TPB>(ln, in, out, tw, st);
-        case 512: return launch_half_M<Ti, T, To, 512, TPB>(ln, in, out, tw, st);
-        case 1024: return launch_half_M<Ti, T, To, 1024, TPB>(ln, in, out, tw, st);
-        case 2048: return launch_half_M<Ti, T, To, 2048, TPB>(ln, in, out, tw, st);
-        case 4096: return launch_half_M<Ti, T, To, 4096, TPB>(ln, in, out, tw, st);
+        case 2: return launch_half_M<Ti, T, To, 2, TPB>(ln, in, out, tw, wide, st);
+        case 4: return launch_half_M<Ti, T, To, 4, TPB>(ln, in, out, tw, wide, st);
+        case 8: return launch_half_M<Ti, T, To, 8, TPB>(ln, in, out, tw, wide, st);
+        case 16: return launch_half_M<Ti, T, To, 16, TPB>(ln, in, out, tw, wide, st);
+        case 32: return launch_half_M<Ti, T, To, 32, TPB>(ln, in, out, tw, wide, st);
+        case 64: return launch_half_M<Ti, T, To, 64, TPB>(ln, in, out, tw, wide, st);
+        case 128: return launch_half_M<Ti, T, To, 128, TPB>(ln, in, out, tw, wide, st);
+        case 256: return launch_half_M<Ti, T, To, 256, TPB>(ln, in, out, tw, wide, st);
+        case 512: return launch_half_M<Ti, T, To, 512, TPB>(ln, in, out, tw, wide, st);
+        case 1024: return launch_half_M<Ti, T, To, 1024, TPB>(ln, in, out, tw, wide, st);
+        case 2048: return launch_half_M<Ti, T, To, 2048, TPB>(ln, in, out, tw, wide, st);
+        case 4096: return launch_half_M<Ti, T, To, 4096, TPB>(ln, in, out, tw, wide, st);
         default: return hipErrorInvalidValue;
         }
     }
@@ -605,13 +625,14 @@ static hipError_t launch_fft_t(const Lines &ln, int odd, const void *in, void *o
 }
 
 // algo bit 0: odd-extension transform (else half length); bit 1: 64-thread
-// blocks (else 256)
+// blocks (else 256); bit 2: 512-thread blocks for strided lines of the
+// half-length transform with 128 <= N+1 <= 256 (tiles of twice the lines)
 template <typename Ti, typename T, typename To>
 static hipError_t launch_dst_t(const Lines &ln, int algo, const void *in, void *out, const void *tw, const void *S,
                                hipStream_t st) {
     if (fft_supported(ln.N)) {
-        if (algo & 2) return launch_fft_t<Ti, T, To, 64>(ln, algo & 1, in, out, tw, st);
-        return launch_fft_t<Ti, T, To, 256>(ln, algo & 1, in, out, tw, st);
+        if (algo & 2) return launch_fft_t<Ti, T, To, 64>(ln, algo, in, out, tw, st);
+        return launch_fft_t<Ti, T, To, 256>(ln, algo, in, out, tw, st);
     }
     if (in == out || S == nullptr) return hipErrorInvalidValue;
     long nlines = ln.ncols * ln.nplanes;

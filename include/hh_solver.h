@@ -60,12 +60,15 @@ size_t hh_context_bytes_in_use(const hh_context *ctx);
 int hh_context_enable_timing(hh_context *ctx, int on);
 /* DST-I of power-of-two N+1.  Bit 0 of algo: 0 = (N+1)-point FFT of the
  * symmetrised row plus a prefix-sum post-pass, 1 = 2(N+1)-point FFT of the
- * odd extension; bit 1: 64-thread blocks instead of 256.  All four are
+ * odd extension; bit 1: 64-thread blocks instead of 256; bit 2: for the
+ * (N+1)-point transform of strided lines with 128 <= N+1 <= 256, 512-thread
+ * blocks (tiles of twice the lines; same bits).  All are
  * orthonormal DST-I to roundoff; the prefix sums of the first make its error
  * grow like sqrt(N) rather than log N (7e-14 relative for a transform and
  * its inverse at N+1 = 4096, against 5e-16).  -1 (default): the
  * (N+1)-point transform, in 64-thread blocks for contiguous rows with
- * N+1 >= 512 and 256-thread blocks otherwise. */
+ * N+1 >= 512, 512-thread blocks for strided lines in FP32 at 128 <= N+1 <=
+ * 256, and 256-thread blocks otherwise. */
 int hh_context_set_dst_algorithm(hh_context *ctx, int algo);
 int hh_context_read_timing(hh_context *ctx, double ms[5], long launches[5], double bytes[5]);
 int hh_context_reset_timing(hh_context *ctx);

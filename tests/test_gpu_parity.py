@@ -46,10 +46,11 @@ def relerr(a, b):
 
 
 # ------------------------------------------------------------------ DST
-@pytest.fixture(params=[-1, 0, 1, 2, 3], ids=["auto", "half", "oddext", "half64", "oddext64"])
+@pytest.fixture(params=[-1, 0, 1, 2, 3, 4], ids=["auto", "half", "oddext", "half64", "oddext64", "halfwidecols"])
 def dctx(ctx, request):
     """The context with each DST variant: (N+1)-point or odd-extension FFT, 256- or 64-thread blocks,
-    and the automatic choice ((N+1)-point, block size by line length)."""
+    512-thread blocks for strided lines, and the automatic choice ((N+1)-point, block size by line
+    length)."""
     ctx.set_dst_algorithm(request.param)
     yield ctx
     ctx.set_dst_algorithm(-1)
