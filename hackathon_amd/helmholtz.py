@@ -86,6 +86,14 @@ def _require_cuda(t, name):
         raise TypeError(f"{name} must be a contiguous complex128 tensor on the GPU")
 
 
+def _require_size(t, name, n, exact=True):
+    """t is a contiguous complex128 GPU tensor of n (exact) or at least n values:
+    the library reads and writes through raw device pointers."""
+    _require_cuda(t, name)
+    if (t.numel() != n) if exact else (t.numel() < n):
+        raise ValueError(f"{name} holds {t.numel()} values, {'expected' if exact else 'needs at least'} {n}")
+
+
 class Context:
     """Device, stream, twiddle tables and scratch of the library."""
 
@@ -136,6 +144,7 @@ class Context:
         """Orthonormal DST-I of every row (last axis) of a complex128 GPU tensor."""
         _require_cuda(x, "x")
         out = x.new_empty(x.shape) if out is None else out
+        _require_size(out, "out", x.numel())
         N = x.shape[-1]
         self.bind_stream()
         _check(lib().hh_dst_rows(self.h, ctypes.c_int(N), ctypes.c_long(x.numel() // N), _dptr(x), _dptr(out)))
@@ -145,6 +154,7 @@ class Context:
         """DST-I along x then y of tensors shaped (..., Ny, Nx)."""
         _require_cuda(x, "x")
         out = x.new_empty(x.shape) if out is None else out
+        _require_size(out, "out", x.numel())
         Ny, Nx = x.shape[-2], x.shape[-1]
         self.bind_stream()
         _check(lib().hh_dst2d_layers(self.h, ctypes.c_int(Nx), ctypes.c_int(Ny),
@@ -228,7 +238,7 @@ class Plan:
         """sol = A^{-1} rhs for a batch of right-hand sides (complex128, GPU)."""
         _require_cuda(rhs, "rhs")
         out = rhs.new_empty(rhs.shape) if out is None else out
-        _require_cuda(out, "out")
+        _require_size(out, "out", rhs.numel())
         self.ctx.bind_stream()
         _check(lib().hh_solve(self.h, _dptr(rhs), _dptr(out), self._batch_of(rhs)))
         return out
@@ -243,8 +253,9 @@ class Plan:
     def residual(self, sol, rhs, out=None):
         """res = A sol - rhs (FP64 compact operator)."""
         _require_cuda(sol, "sol")
-        _require_cuda(rhs, "rhs")
+        _require_size(rhs, "rhs", sol.numel())
         out = sol.new_empty(sol.shape) if out is None else out
+        _require_size(out, "out", sol.numel())
         self.ctx.bind_stream()
         _check(lib().hh_residual(self.h, _dptr(sol), _dptr(rhs), _dptr(out), self._batch_of(sol)))
         return out
@@ -309,10 +320,18 @@ class Plan2D(Plan):
 
     def rhs(self, order, hx, hy, f_e, u_e, k2e, fy_b=None, fxx_b=None, fyy_b=None, k2yb=None, out=None):
         """Compact right-hand side (rhs.c) from f and u on the extended grid."""
+        ext = (self.Nx + 2) * (self.Ny + 2)
         _require_cuda(f_e, "f_e")
-        _require_cuda(u_e, "u_e")
-        batch = f_e.numel() // ((self.Nx + 2) * (self.Ny + 2))
+        if f_e.numel() % ext:
+            raise ValueError(f"f_e holds {f_e.numel()} values, not a whole number of extended grids of {ext}")
+        batch = f_e.numel() // ext
+        _require_size(u_e, "u_e", f_e.numel())
         out = f_e.new_empty((batch, self.Ny, self.Nx)) if out is None else out
+        _require_size(out, "out", batch * self.Nx * self.Ny)
+        ring = 2 * (self.Nx + 2) + 2 * (self.Ny + 2)
+        for t, name, n in ((fy_b, "fy_b", 2 * self.Nx), (fxx_b, "fxx_b", ring), (fyy_b, "fyy_b", ring)):
+            if t is not None:
+                _require_size(t, name, batch * n, exact=False)
         # one k^2 profile (and (k^2)_y pair) per operator; a single one is shared
         k2e = _c128(k2e)
         if k2e.size == self.Ny + 2:
@@ -351,10 +370,17 @@ class Plan3D(Plan):
         self.precision = precision
 
     def rhs(self, order, hx, hy, hz, f_e, u_e, k2e, fxx_f=None, fyy_f=None, fzz_f=None, k2d=None, out=None):
+        X, Y, Z = self.Nx + 2, self.Ny + 2, self.Nz + 2
         _require_cuda(f_e, "f_e")
-        _require_cuda(u_e, "u_e")
-        batch = f_e.numel() // ((self.Nx + 2) * (self.Ny + 2) * (self.Nz + 2))
+        if f_e.numel() % (X * Y * Z):
+            raise ValueError(f"f_e holds {f_e.numel()} values, not a whole number of extended grids of {X * Y * Z}")
+        batch = f_e.numel() // (X * Y * Z)
+        _require_size(u_e, "u_e", f_e.numel())
         out = f_e.new_empty((batch, self.Nz, self.Ny, self.Nx)) if out is None else out
+        _require_size(out, "out", batch * self.Nx * self.Ny * self.Nz)
+        for t, name, n in ((fxx_f, "fxx_f", 2 * Y * Z), (fyy_f, "fyy_f", 2 * X * Z), (fzz_f, "fzz_f", 2 * X * Y)):
+            if t is not None:
+                _require_size(t, name, batch * n, exact=False)
         k2e = _c128(k2e)
         if k2e.size == self.Nz + 2:
             k2e = np.tile(k2e, self.nop)
@@ -442,7 +468,8 @@ class Server:
     def submit(self, dim, N, order, k, rhs, precision=PREC_F64, h=None, out=None):
         """Queue one solve; `out` (optional, e.g. pinned memory for fast copies) receives the solution."""
         Nx, Ny, Nz = (N, N, N) if np.isscalar(N) else (tuple(N) + (1,))[:3]
-        rhs = _c128(rhs)
+        n = Nx * Ny * (Nz if dim == 3 else 1)
+        rhs = _c128(rhs, n)   # the server copies whole fields from these buffers
         sol = np.empty_like(rhs) if out is None else out
         if sol.dtype != np.complex128 or sol.size != rhs.size or not sol.flags.c_contiguous:
             raise ValueError("out must be a contiguous complex128 array of the right-hand side's size")
