@@ -1,0 +1,122 @@
+"""Krylov mode: the direct solver as the preconditioner of a restarted GMRES
+for operators it does not diagonalise itself.
+
+The DST diagonalisation needs coefficients that vary along the tridiagonal
+axis only (y in 2D).  A wave number that also varies along x, k(x, y),
+breaks it; the direct solver for the x-averaged k^2 is then a close
+approximation of the operator, and right-preconditioned GMRES converges
+in a handful of iterations (the reference's paper names this use, the
+solver as a caller's inner apply: paper.pdf p.2 l.127-139).
+
+The operator apply and the preconditioner are the library's HIP kernels
+(the compact residual kernel plus a diagonal term, and the direct solve);
+the Krylov vector algebra (dot products, updates of a few vectors) runs in
+PyTorch on the same device and stream.
+"""
+import numpy as np
+
+from . import helmholtz as H
+
+
+def gmres(apply_a, b, precond=None, tol=1e-10, restart=30, maxiter=300):
+    """Solve A x = b with right-preconditioned restarted GMRES(restart).
+
+    apply_a(v) -> A v and precond(v) -> M^{-1} v take and return complex128
+    GPU tensors of b's shape.  Stops when ||b - A x||_2 <= tol ||b||_2 (the
+    true residual, recomputed at every restart) or after maxiter inner
+    iterations.  Returns (x, info) with info = {iterations, rel_residual,
+    converged}."""
+    torch = H._torch()
+    M = precond if precond is not None else (lambda v: v)
+    nb = float(torch.linalg.vector_norm(b))
+    x = torch.zeros_like(b)
+    if nb == 0.0:
+        return x, {"iterations": 0, "rel_residual": 0.0, "converged": True}
+    r = b.clone()
+    rel = 1.0
+    it = 0
+    while True:
+        beta = float(torch.linalg.vector_norm(r))
+        rel = beta / nb
+        if rel <= tol or it >= maxiter:
+            break
+        V = [r / beta]
+        Hm = np.zeros((restart + 1, restart), np.complex128)
+        cs = np.zeros(restart, np.complex128)
+        sn = np.zeros(restart, np.complex128)
+        g = np.zeros(restart + 1, np.complex128)
+        g[0] = beta
+        k = 0
+        for j in range(restart):
+            w = apply_a(M(V[j]))
+            for i in range(j + 1):   # modified Gram-Schmidt
+                hij = complex(torch.vdot(V[i].reshape(-1), w.reshape(-1)))
+                Hm[i, j] = hij
+                w = w - hij * V[i]
+            hn = float(torch.linalg.vector_norm(w))
+            Hm[j + 1, j] = hn
+            for i in range(j):       # earlier Givens rotations on the new column
+                t = cs[i] * Hm[i, j] + sn[i] * Hm[i + 1, j]
+                Hm[i + 1, j] = -np.conj(sn[i]) * Hm[i, j] + cs[i] * Hm[i + 1, j]
+                Hm[i, j] = t
+            # rotation [[c, s], [-conj(s), c]] (c real) zeroing the subdiagonal
+            a = Hm[j, j]
+            den = np.hypot(abs(a), hn)
+            if abs(a) == 0.0:
+                cs[j], sn[j] = 0.0, 1.0
+            else:
+                cs[j] = abs(a) / den
+                sn[j] = (a / abs(a)) * hn / den
+            Hm[j, j] = cs[j] * a + sn[j] * hn
+            Hm[j + 1, j] = 0.0
+            g[j + 1] = -np.conj(sn[j]) * g[j]
+            g[j] = cs[j] * g[j]
+            it += 1
+            k = j + 1
+            if hn == 0.0 or abs(g[j + 1]) / nb <= tol or it >= maxiter:
+                break
+            V.append(w / hn)
+        y = np.linalg.solve(np.triu(Hm[:k, :k]), g[:k])
+        upd = V[0] * complex(y[0])
+        for i in range(1, k):
+            upd = upd + V[i] * complex(y[i])
+        x = x + M(upd)
+        r = b - apply_a(x)
+    return x, {"iterations": it, "rel_residual": rel, "converged": rel <= tol}
+
+
+class VariableK2D:
+    """Second-order five-point Helmholtz operator on the unit square with a
+    wave number varying in x and y (homogeneous Dirichlet boundary):
+
+        (A u)_ij = u_{i-1,j} + u_{i+1,j} + u_{i,j-1} + u_{i,j+1}
+                   + (h^2 k^2(x_i, y_j) - 4) u_ij
+
+    (the reference's second-order stencil, coefficients.c:9-23, with k^2
+    taken per point).  k2 is an (N, N) array of k^2 at the interior points,
+    [j][i] = k^2(x_i, y_j).  The preconditioner is the direct solve of the
+    same stencil with k^2 averaged along x; A u is its residual kernel plus
+    the diagonal h^2 (k^2 - mean_x k^2) u."""
+
+    def __init__(self, ctx, N, k2):
+        torch = H._torch()
+        k2 = np.asarray(k2, dtype=np.complex128).reshape(N, N)
+        self.N, self.h = N, 1.0 / (N + 1)
+        kbar = k2.mean(axis=1)                                   # per row j
+        k2e = np.concatenate([[kbar[0]], kbar, [kbar[-1]]])      # extended rows (only interior rows enter)
+        coef = H.coefficients2d(2, N, N, self.h, self.h, k2e)
+        self.plan = H.Plan2D(ctx, N, N, coef)
+        d = self.h ** 2 * (k2 - kbar[:, None])
+        self.diag = torch.from_numpy(np.ascontiguousarray(d)).to("cuda")
+        self.zero = torch.zeros((N, N), dtype=torch.complex128, device="cuda")
+
+    def apply(self, u):
+        u = u.reshape(self.N, self.N)
+        return self.plan.residual(u, self.zero) + self.diag * u
+
+    def precond(self, v):
+        return self.plan.solve(v.reshape(self.N, self.N).contiguous())
+
+    def solve(self, F, tol=1e-12, restart=30, maxiter=300):
+        """x with A x = F (F: (N, N) complex128 GPU tensor)."""
+        return gmres(self.apply, F.reshape(self.N, self.N).contiguous(), self.precond, tol, restart, maxiter)
