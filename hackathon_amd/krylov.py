@@ -5,8 +5,9 @@ The DST diagonalisation needs coefficients that vary along the tridiagonal
 axis only (y in 2D).  A wave number that also varies along x, k(x, y),
 breaks it; the direct solver for the x-averaged k^2 is then a close
 approximation of the operator, and right-preconditioned GMRES converges
-in a handful of iterations (the reference's paper names this use, the
-solver as a caller's inner apply: paper.pdf p.2 l.127-139).
+in a handful of iterations (the reference's paper cites this use of
+the direct solver, as a preconditioner for nonconstant-coefficient and
+Sommerfeld-like problems: Resources/paper.pdf p.1, l.27-41).
 
 The operator apply and the preconditioner are the library's HIP kernels
 (the compact residual kernel plus a diagonal term, and the direct solve);
