@@ -239,7 +239,7 @@ HH_DEV long xcd_tile(long b, long G) {
 // right-hand sides o + nop (NR (y / nop) + r), r < NR, those past the batch
 // are computed on a clamped index and not stored.
 template <typename T, int MMAX, int NR, int NT, class Prob>
-__global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P, int batch, int nop) {
+__global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P, int batch, int nop, int g0) {
     using OT = typename Prob::OT;
     using O = OpAr<T, OT>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
     const int w = threadIdx.x % W, c = threadIdx.x / W;
     const long m = xcd_tile(blockIdx.x, gridDim.x) * W + w;
     const bool valid = (c < P) && (m < nmodes);
-    const int o = blockIdx.y % nop, jg = blockIdx.y / nop;
+    const int o = blockIdx.y % nop, jg = g0 + (int)(blockIdx.y / nop);
     int bidx[NR];
     bool bok[NR];
 #pragma unroll
@@ -475,9 +475,17 @@ static hipError_t launch_tri_nr(const Prob &pb, int n, long nmodes, int batch, i
     }
     const int per_op = (batch + nop - 1) / nop;
     const int groups = (per_op + NR - 1) / NR;
-    dim3 grid((unsigned)((nmodes + W - 1) / W), (unsigned)(nop * groups));
-    hipLaunchKernelGGL(kfn, grid, dim3(threads), shm, st, pb, nmodes, W, P, batch, nop);
-    return hipGetLastError();
+    // grid y holds at most 65535 blocks: launch the right-hand-side groups in slices
+    if (nop > 65535) return hipErrorInvalidValue;
+    const int gmax = 65535 / nop;
+    for (int g0 = 0; g0 < groups; g0 += gmax) {
+        const int ng = groups - g0 < gmax ? groups - g0 : gmax;
+        dim3 grid((unsigned)((nmodes + W - 1) / W), (unsigned)(nop * ng));
+        hipLaunchKernelGGL(kfn, grid, dim3(threads), shm, st, pb, nmodes, W, P, batch, nop, g0);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 template <typename T, int MM, class Prob>

@@ -186,6 +186,25 @@ def test_batch_alone_equals_in_batch(H, ctx, oracle, N):
         assert np.array_equal(alone, allsol[b])
 
 
+def test_batch_beyond_one_grid_dimension(H, ctx, oracle):
+    """70000 right-hand sides in one solve: more groups than one launch's
+    grid holds (65535), so the mode solves run in slices; right-hand sides
+    in both slices give the bits they give alone."""
+    import torch
+    N, B = 7, 70000
+    s = oracle.solve2d(4, N, 1, [5.0, 0.0, 1, 2, 0.3], threads=8)
+    plan = H.Plan2D(ctx, N, N, s["coef"])
+    g = torch.Generator(device="cuda").manual_seed(11)
+    rhs = torch.randn((B, N, N), dtype=torch.complex128, device="cuda", generator=g)
+    rhs[B - 1] = dev(s["rhs"]).reshape(N, N)
+    say(f"batch of {B} right-hand sides, N={N}")
+    sol = plan.solve(rhs)
+    for b in (0, 65534, 65535, 69998):
+        alone = plan.solve(rhs[b:b + 1].contiguous())
+        assert torch.equal(alone[0], sol[b]), f"right-hand side {b} differs alone"
+    assert relerr(host(sol[B - 1]).ravel(), s["sol"]) <= 1e-12
+
+
 # ------------------------------------------------------------------ 3D solve
 CASES3D = [
     (2, 0, [10, 9, 10, 9], 15), (4, 0, [10, 9, 10, 9], 31), (6, 0, [10, 9, 10, 9], 31),
