@@ -1,7 +1,7 @@
 /*
  * CPU FP64 oracle for the compact-scheme Helmholtz direct solver.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker that the GPU
+ * TEST INFRASTRUCTURE ONLY.  This library is the oracle that the GPU
  * library is compared against.  Only the test suite, the smoke check and the
  * CPU-baseline leg of the benchmark may load it; the product never does.
  *
