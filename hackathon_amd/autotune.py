@@ -7,8 +7,9 @@ device with the library's own kernel timers (HIP events on the launch
 stream).  Results are kept in a JSON table keyed by
 (dimension, system length, modes, batch bucket, precision) so that a shape
 does not depend on what else a batch contains; `apply_tuned(plan, batch)`
-sets the tabulated shape on a plan.  The four DST variants (half-length or
-odd-extension transform, 256- or 64-thread blocks) are timed as well.
+sets the tabulated shape on a plan.  The DST variants (half-length or
+odd-extension transform, 256-, 64- or, for strided lines, 512-thread
+blocks, and the automatic choice) are timed as well.
 
     python -m hackathon_amd.autotune --dim 2 --N 1023 --batch 64
 """
@@ -137,7 +138,9 @@ def main():
     print(f"tridiagonal shapes, dim={args.dim} N={N} batch={args.batch} precision={args.precision}", flush=True)
     res = tune(plan, rhs, out, verbose=True)
     print(json.dumps({k: v for k, v in res.items() if k != "all"}), flush=True)
-    for algo in (0, 1, 2, 3):
+    # -1 automatic; bit 0 odd extension, bit 1 one-wave blocks, bit 2 512-thread
+    # blocks for strided lines (N+1 in 128..256)
+    for algo in (-1, 0, 1, 2, 3, 4):
         ctx.set_dst_algorithm(algo)
         _, per = time_tri(plan, rhs, out)
         print(f"DST algorithm {algo}: {1e3 * per[0]:9.1f} us per launch", flush=True)
