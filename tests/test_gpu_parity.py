@@ -289,7 +289,7 @@ def test_rhs3d(H, ctx, oracle, order, kind, p, N):
 
 
 # ------------------------------------------------------------------ mixed precision
-@pytest.mark.parametrize("dim,order,N", [(2, 6, 255), (2, 4, 1023), (2, 6, 2047), (3, 6, 63)])
+@pytest.mark.parametrize("dim,order,N", [(2, 6, 255), (2, 4, 1023), (2, 6, 2047), (3, 6, 63), (3, 4, 127)])
 def test_mixed_precision(H, ctx, oracle, dim, order, N):
     say(f"mixed dim={dim} order={order} N={N}")
     if dim == 2:
