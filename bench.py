@@ -92,10 +92,20 @@ def manufactured(torch, N, B, kk, seed, dev):
     return dict(a=a, b=b, phi=phi, f_e=f_e, u_e=u_e, fy_b=fy_b, fxx_b=fxx_b, fyy_b=fyy_b)
 
 
-def cpu_baseline(O, N, coef, F_host, seconds):
-    """Oracle restatement of the reference 2D solver() on this host's cores."""
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    threads = max(1, min(threads, 16))
+def host_threads():
+    """Cores this process may use (capped at 16, the share of one GPU's host)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(n, 16))
+
+
+def host_cpus():
+    return {"online": os.cpu_count(), "usable": host_threads()}
+
+
+def cpu_baseline(O, N, coef, F_host, seconds, threads=None):
+    """Oracle restatement of the reference 2D solver() on this host's cores
+    (threads=1: serial, as the reference runs without -DUSE_OMP)."""
+    threads = host_threads() if threads is None else threads
     LUU = O.lu2d(N, N, coef, threads)
     done, t_used = 0, 0.0
     while t_used < seconds:   # cycle over the right-hand sides until the budget is spent
@@ -104,7 +114,8 @@ def cpu_baseline(O, N, coef, F_host, seconds):
         t_used += time.perf_counter() - t0
         done += 1
     return {"value": done * N * N / t_used, "unit": "complex DOF/s", "cores": threads, "kind": "port",
-            "sample": f"oracle solver() (own FFT DST, plan creation inside the call, OpenMP over rows/modes) on "
+            "sample": f"oracle solver() (own FFT DST, plan creation inside the call, "
+                      f"{'serial' if threads == 1 else 'OpenMP over rows/modes'}) on "
                       f"{done} 6th-order right-hand sides of the same N={N} k=50 workload, {t_used:.1f} s"}
 
 
@@ -199,12 +210,12 @@ def run_3d(args, torch, H):
     for name, plan, out in paths:
         m, d = ctx.norms(out, u_exact)
         res[name]["disc_err_vs_exact"] = d / max(m, 1e-300)
-    cpu = None
+    cpu = cpu_serial = None
     if not (args.no_check and args.no_cpu):
         from oracle import oracle as O
         log("3d: oracle solve on the host")
         Fh = F.cpu().numpy().ravel()
-        threads = max(1, min(len(os.sched_getaffinity(0)), 16))
+        threads = host_threads()
         t0 = time.perf_counter()
         LUU = O.lu3d(N, N, N, np.concatenate(coef))
         t_lu = time.perf_counter() - t0
@@ -213,16 +224,20 @@ def run_3d(args, torch, H):
             g = out.cpu().numpy().ravel()
             res[name]["max_rel_err_vs_oracle"] = float(np.abs(g - ref).max() / np.abs(ref).max())
         if not args.no_cpu:
-            done, t_used = 0, 0.0
-            while t_used < args.cpu_seconds:   # solver() only, as the reference times it
-                t0 = time.perf_counter()
-                O.solver3d(N, N, N, Fh, LUU, threads)
-                t_used += time.perf_counter() - t0
-                done += 1
-            cpu = {"value": done * N ** 3 / t_used, "unit": "complex DOF/s", "cores": threads, "kind": "port",
-                   "sample": f"oracle solver() of the 3D reference CPU path (OpenMP over layers/modes), {done} solves "
-                             f"of the same {N}^3 k=30 right-hand side, {t_used:.1f} s (factorisation LU() "
-                             f"{t_lu:.1f} s, untimed like the reference)"}
+            def sample(nthreads, seconds):
+                done, t_used = 0, 0.0
+                while t_used < seconds:   # solver() only, as the reference times it
+                    t0 = time.perf_counter()
+                    O.solver3d(N, N, N, Fh, LUU, nthreads)
+                    t_used += time.perf_counter() - t0
+                    done += 1
+                how = "serial, as the reference's 3D CPU solver" if nthreads == 1 else "OpenMP over layers/modes"
+                return {"value": done * N ** 3 / t_used, "unit": "complex DOF/s", "cores": nthreads, "kind": "port",
+                        "sample": f"oracle solver() of the 3D reference CPU path ({how}), {done} solves of the same "
+                                  f"{N}^3 k=30 right-hand side, {t_used:.1f} s (factorisation LU() {t_lu:.1f} s, "
+                                  f"untimed like the reference)"}
+            cpu = sample(threads, args.cpu_seconds)
+            cpu_serial = sample(1, args.cpu_seconds / 2)
     names = [p[0] for p in paths]
     ok = {k: res[k].get("max_rel_err_vs_oracle", 0.0) <= 1e-10 for k in names}
     best = max((k for k in names if ok[k]), key=lambda k: res[k]["dof_per_s"], default="fp64")
@@ -235,7 +250,7 @@ def run_3d(args, torch, H):
                      dtype="f32+f64" if best != "fp64" else "f64", value_path=best,
                      p50_ms=res[best]["p50_ms"], p99_ms=res[best]["p99_ms"],
                      mixed_over_fp64=res[best_mixed]["dof_per_s"] / res["fp64"]["dof_per_s"], results=res,
-                     cpu_baseline=cpu)
+                     cpu_baseline=cpu, cpu_baseline_serial=cpu_serial, host_cpus=host_cpus())
 
 
 def run_uneven(args, torch, H):
@@ -526,6 +541,7 @@ def main():
     # correctness: sampled results against the CPU FP64 oracle
     check = {}
     cpu = None
+    cpu_serial = None
     if not (args.no_check and args.no_cpu):
         from oracle import oracle as O
         for o in orders:
@@ -545,6 +561,7 @@ def main():
             if not args.no_cpu and o == orders[-1]:
                 Fall = F[o].cpu().numpy().reshape(B, N * N)
                 cpu = cpu_baseline(O, N, coef, Fall, args.cpu_seconds)
+                cpu_serial = cpu_baseline(O, N, coef, Fall, args.cpu_seconds / 2, threads=1)
 
     out = {
         "metric": "complex DOF/s at <=1e-10 rel err vs FP64 oracle; p50/p99 solve latency",
@@ -573,6 +590,8 @@ def main():
                      "bytes_per_dof_per_pass": 32},
         "solve_level_frac": value * 32 / (HBM_PEAK_GBS * 1e9),
         "cpu_baseline": cpu,
+        "cpu_baseline_serial": cpu_serial,
+        "host_cpus": host_cpus(),
     }
     print(json.dumps(out), flush=True)
     if args.copy_probe > 0:
