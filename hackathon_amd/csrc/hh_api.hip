@@ -548,6 +548,84 @@ extern "C" int hh_coefficients3d(int order, int Nx, int Ny, int Nz, double hx, d
     return HH_OK;
 }
 
+// ------------------------------------------------------------------ Robin closure
+// Boundary condition du/dn + alpha u = g on the first (lo) or last (hi) row of
+// the tridiagonal axis, the row itself lying on the boundary.  The central
+// difference (u_1 - u_{-1}) / 2h places a ghost row u_{-1} = u_1 - 2h alpha u_0
+// + 2h g outside the grid; substituting it into the boundary row's stencil
+// M u_{-1} + C u_0 + P u_1 = F_0 gives (C - 2h alpha M) u_0 + (P + M) u_1 =
+// F_0 - 2h M g (mirrored on the hi side).  M, C and P are the same
+// combinations of the DST-direction neighbour sums in every row, so the fold
+// is band-wise on the coefficient vectors, and the DST still diagonalises
+// the folded operator.  corr receives -2h M g (the row of F to add).
+// bands: pointers to nb centre, nb plus and nb minus vectors, in that order.
+static void robin_fold(int n, double h, cd alpha, bool lo, double *const *ctr, double *const *pls,
+                       double *const *mns, int nb, cd *orig) {
+    const int r = lo ? 0 : n - 1;
+    for (int q = 0; q < nb; q++) {
+        const cd m = ld(mns[q], r), p = ld(pls[q], r), c = ld(ctr[q], r);
+        const cd out = lo ? m : p;             // band that reached the ghost row
+        orig[q] = out;
+        st(ctr[q], r, c - 2.0 * h * alpha * out);
+        if (lo) { st(pls[q], r, p + m); st(mns[q], r, cd(0)); }
+        else    { st(mns[q], r, m + p); st(pls[q], r, cd(0)); }
+    }
+}
+
+extern "C" int hh_robin2d(int Nx, int Ny, double h, const double *alpha_lo, const double *g_lo,
+                          const double *alpha_hi, const double *g_hi, double *const coef[6], double *corr_lo,
+                          double *corr_hi) {
+    if (Nx < 1 || Ny < 2 || !(h > 0) || !coef) return fail(HH_ERR_ARG, "hh_robin2d: bad arguments");
+    for (int q = 0; q < 6; q++)
+        if (!coef[q]) return fail(HH_ERR_ARG, "hh_robin2d: null coefficient array %d", q);
+    // [bp, ap, b, a, bm, am]: centre (b, a), plus (bp, ap), minus (bm, am); per band the x-operator
+    // is (first) * (u_{i-1} + u_{i+1}) + (second) * u_i
+    double *ctr[2] = {coef[2], coef[3]}, *pls[2] = {coef[0], coef[1]}, *mns[2] = {coef[4], coef[5]};
+    for (int side = 0; side < 2; side++) {
+        const double *al = side == 0 ? alpha_lo : alpha_hi, *g = side == 0 ? g_lo : g_hi;
+        double *corr = side == 0 ? corr_lo : corr_hi;
+        if (!al) continue;
+        cd o[2];
+        robin_fold(Ny, h, cd(al[0], al[1]), side == 0, ctr, pls, mns, 2, o);
+        if (!corr) continue;
+        for (int i = 0; i < Nx; i++) {
+            const cd nb = (i > 0 ? ld(g, i - 1) : cd(0)) + (i + 1 < Nx ? ld(g, i + 1) : cd(0));
+            st(corr, i, -2.0 * h * (o[0] * nb + o[1] * ld(g, i)));
+        }
+    }
+    return HH_OK;
+}
+
+extern "C" int hh_robin3d(int Nx, int Ny, int Nz, double h, const double *alpha_lo, const double *g_lo,
+                          const double *alpha_hi, const double *g_hi, double *const coef[12], double *corr_lo,
+                          double *corr_hi) {
+    if (Nx < 1 || Ny < 1 || Nz < 2 || !(h > 0) || !coef) return fail(HH_ERR_ARG, "hh_robin3d: bad arguments");
+    for (int q = 0; q < 12; q++)
+        if (!coef[q]) return fail(HH_ERR_ARG, "hh_robin3d: null coefficient array %d", q);
+    // [a, b, c, d | ap..dp | am..dm]: per band the (x, y)-operator is a * (diagonal neighbours)
+    // + b * (x neighbours) + c * (y neighbours) + d * u
+    double *ctr[4] = {coef[0], coef[1], coef[2], coef[3]}, *pls[4] = {coef[4], coef[5], coef[6], coef[7]},
+           *mns[4] = {coef[8], coef[9], coef[10], coef[11]};
+    auto G = [&](const double *g, int i, int j) {
+        return (i >= 0 && i < Nx && j >= 0 && j < Ny) ? ld(g, i + (long)j * Nx) : cd(0);
+    };
+    for (int side = 0; side < 2; side++) {
+        const double *al = side == 0 ? alpha_lo : alpha_hi, *g = side == 0 ? g_lo : g_hi;
+        double *corr = side == 0 ? corr_lo : corr_hi;
+        if (!al) continue;
+        cd o[4];
+        robin_fold(Nz, h, cd(al[0], al[1]), side == 0, ctr, pls, mns, 4, o);
+        if (!corr) continue;
+        for (int j = 0; j < Ny; j++)
+            for (int i = 0; i < Nx; i++) {
+                const cd dg = G(g, i - 1, j - 1) + G(g, i + 1, j - 1) + G(g, i - 1, j + 1) + G(g, i + 1, j + 1);
+                const cd xg = G(g, i - 1, j) + G(g, i + 1, j), yg = G(g, i, j - 1) + G(g, i, j + 1);
+                st(corr, i + (long)j * Nx, -2.0 * h * (o[0] * dg + o[1] * xg + o[2] * yg + o[3] * G(g, i, j)));
+            }
+    }
+    return HH_OK;
+}
+
 // ------------------------------------------------------------------ plans
 // Device band-table order from the caller's operator vectors.
 // 2D: input [op][bp, ap, b, a, bm, am][Ny] -> table [op][q](bm, am, b, a, bp, ap);

@@ -200,6 +200,44 @@ def coefficients3d(order, Nx, Ny, Nz, hx, hy, hz, k2e, k2d=None):
     return out
 
 
+def _robin(fn, dims, nvec, coef, h, lo, hi):
+    n_tri = dims[-1]
+    face = int(np.prod(dims[:-1]))
+    out = [_c128(v, n_tri).copy() for v in _operators(coef, nvec, n_tri)[0]]
+    sides, corr = [], []
+    for s in (lo, hi):
+        if s is None:
+            sides += [ctypes.c_void_p(0), ctypes.c_void_p(0)]
+            corr.append(None)
+            continue
+        alpha, g = s
+        a = np.array([complex(alpha)], np.complex128)
+        gg = None if g is None else _c128(g, face)
+        c = np.zeros(face, np.complex128)
+        corr.append((a, gg, c))
+        sides += [_hptr(a), _hptr(gg) if gg is not None else ctypes.c_void_p(0)]
+    arr = (ctypes.c_void_p * nvec)(*[o.ctypes.data for o in out])
+    cptr = [_hptr(c[2]) if c is not None else ctypes.c_void_p(0) for c in corr]
+    _check(fn(*dims, ctypes.c_double(h), *sides, arr, *cptr))
+    shape = tuple(reversed(dims[:-1]))
+    return out, [None if c is None else c[2].reshape(shape) for c in corr]
+
+
+def robin2d(coef, Nx, Ny, h, lo=None, hi=None):
+    """Robin boundary du/dn + alpha u = g on the first (lo) and/or last (hi)
+    row of y, which then lie on the boundary; lo / hi = (alpha, g) with g
+    the boundary data along x (Nx complex, None = 0), or None (Dirichlet).
+    Returns (folded coefficient vectors, [corr_lo, corr_hi]): add corr to
+    that row of the right-hand side.  Second-order closure (ghost-row
+    elimination with central differences); no reference counterpart."""
+    return _robin(lib().hh_robin2d, (Nx, Ny), 6, coef, h, lo, hi)
+
+
+def robin3d(coef, Nx, Ny, Nz, h, lo=None, hi=None):
+    """As robin2d on the first/last z-layer; g: (Ny, Nx) face data."""
+    return _robin(lib().hh_robin3d, (Nx, Ny, Nz), 12, coef, h, lo, hi)
+
+
 # ------------------------------------------------------------------ plans
 def _operators(coef, nvec, n):
     """Normalise one operator (nvec vectors of length n) or a list of them."""

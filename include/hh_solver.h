@@ -90,6 +90,26 @@ int hh_coefficients2d(int order, int Nx, int Ny, double hx, double hy, const dou
 int hh_coefficients3d(int order, int Nx, int Ny, int Nz, double hx, double hy, double hz,
                       const double *k2e, const double *k2d, double *const coef[12]);
 
+/* ---- Robin / Sommerfeld closure of the tridiagonal axis (host, O(N) / O(N^2)) ----
+ * No counterpart in the reference, which solves Dirichlet problems only
+ * (2D rhs.c:126-148, 3D rhs.c:167-174); paper.pdf p.2 eq.(2) names such
+ * boundaries.  The first (lo) and/or last (hi) row of the tridiagonal axis
+ * (y in 2D, z in 3D) lies on the boundary, where du/dn + alpha u = g (n the
+ * outward normal; alpha = -i k gives the first-order Sommerfeld condition).
+ * The ghost row of the central difference is eliminated into the band
+ * vectors, in place: coef as produced by hh_coefficients2d/3d (2D: bp, ap,
+ * b, a, bm, am; 3D: the twelve vectors).  corr_lo / corr_hi (Nx or Nx*Ny
+ * complex, NULL = not wanted) receive the term to add to that boundary row
+ * of the right-hand side, -2h (ghost-row stencil) g.  alpha_* (one complex)
+ * NULL = that side stays Dirichlet; g_* NULL = zero data.  The closure is
+ * second-order accurate whatever the interior scheme.  Needs Ny (Nz) >= 2. */
+int hh_robin2d(int Nx, int Ny, double h, const double *alpha_lo, const double *g_lo,
+               const double *alpha_hi, const double *g_hi, double *const coef[6], double *corr_lo,
+               double *corr_hi);
+int hh_robin3d(int Nx, int Ny, int Nz, double h, const double *alpha_lo, const double *g_lo,
+               const double *alpha_hi, const double *g_hi, double *const coef[12], double *corr_lo,
+               double *corr_hi);
+
 /* ---- plans: one operator, device-resident coefficients and eigenvalues ----
  * Replace the precomputation of LU() (2D_Program/main_functions/LU.c:3-46,
  * 3D_Program/main_functions/LU.c:4-81): no N^2/N^3 factors are stored, the
