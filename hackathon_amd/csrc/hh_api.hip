@@ -905,10 +905,13 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
         hh::tri_shape(n, nmodes, hh::F64, &t.W, &t.P, &t.mmax);
     }
     t.nr = 1;
-    // Real bands, FP64, 16 rows per thread: two right-hand sides per thread
-    // share the operator's pivots (measured at N=1023 x 64: 590 against 654
-    // us per launch) when the grid still has 8 blocks per CU or more.
-    if (t.real && tri_comp == 0 && tc == 0 && t.mmax == 16) {
+    // Real bands, FP64 elimination, 16 rows per thread: two right-hand sides
+    // per thread share the operator's pivots (measured at N=1023 x 64: 590
+    // against 654 us per launch in FP64; 419 against 531 us with FP32
+    // storage, where an operator must have two right-hand sides to pair)
+    // when the grid still has 8 blocks per CU or more.
+    const long per_op = (batch + p->nop - 1) / p->nop;
+    if (t.real && tri_comp == 0 && t.mmax == 16 && (tc == 0 || per_op >= 2)) {
         const long tiles = (nmodes + t.W - 1) / t.W;
         const long groups = ((batch + p->nop - 1) / p->nop + 1) / 2;
         if (tiles * p->nop * groups >= 2048) t.nr = 2;

@@ -143,6 +143,27 @@ def test_rhs_per_thread_same_bits(H, ctx, oracle, nop, batch):
         assert relerr(sol[b].ravel(), probs[b % nop]["sol"]) <= 1e-12
 
 
+def test_mixed_rhs_per_thread_same_bits(H, ctx, oracle):
+    """FP32 storage with FP64 elimination (mixed precision): pairing two
+    right-hand sides per thread, as the built-in shape does for a batch with
+    two or more per operator, gives the same bits as one per thread."""
+    import torch
+    N = 1023
+    s = oracle.solve2d(6, N, 1, [50.0, 0.0, 4, 9, 0.7], threads=8)
+    plan = H.Plan2D(ctx, N, N, s["coef"], precision=H.PREC_MIXED)
+    plan.set_refine(0.0, 2)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    rhs = torch.randn((5, N, N), dtype=torch.complex128, device="cuda", generator=g)
+    rhs[0] = dev(s["rhs"]).reshape(N, N)
+    outs = []
+    for shape in ((0, 0, 0), (4, 16, 1), (4, 16, 2)):
+        H._check(H.lib().hh_plan_set_tri_shape(plan.h, *shape))
+        outs.append(plan.solve(rhs).cpu().numpy())
+        say(f"mixed N={N} batch=5 shape={shape}")
+    assert np.array_equal(outs[0], outs[2]) and np.array_equal(outs[1], outs[2])
+    assert relerr(outs[0][0].ravel(), s["sol"]) <= 1e-10
+
+
 def test_spectrum_layouts_same_bits(H, ctx, oracle):
     """Row-major and mode-tiled spectra (any tile width) give the same bits."""
     import torch
