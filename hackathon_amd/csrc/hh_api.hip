@@ -264,7 +264,9 @@ static int dst_algo_for(const hh_context *c, int N) {
 // slower, 120 against 116 us).
 static int dst_algo_lines(const hh_context *c, const hh::Lines &ln, int tc) {
     int algo = dst_algo_for(c, ln.N);
-    if (c->dst_algo < 0 && ln.elem_stride == 1 && !ln.in_tw && !ln.out_tw && ln.N + 1 >= 512) algo |= 2;
+    // one-wave blocks pay for FP64 rows only (N+1 = 1024: 472 against 521 us); with FP32 storage
+    // 256-thread blocks stay faster (352 against 393 us per pass at 1023^2 x 64)
+    if (c->dst_algo < 0 && !tc && ln.elem_stride == 1 && !ln.in_tw && !ln.out_tw && ln.N + 1 >= 512) algo |= 2;
     if (c->dst_algo < 0 && ln.elem_stride != 1 && tc && ln.N + 1 >= 128 && ln.N + 1 <= 256) algo |= 4;
     return algo;
 }

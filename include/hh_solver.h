@@ -66,7 +66,7 @@ int hh_context_enable_timing(hh_context *ctx, int on);
  * orthonormal DST-I to roundoff; the prefix sums of the first make its error
  * grow like sqrt(N) rather than log N (7e-14 relative for a transform and
  * its inverse at N+1 = 4096, against 5e-16).  -1 (default): the
- * (N+1)-point transform, in 64-thread blocks for contiguous rows with
+ * (N+1)-point transform, in 64-thread blocks for contiguous FP64 rows with
  * N+1 >= 512, 512-thread blocks for strided lines in FP32 at 128 <= N+1 <=
  * 256, and 256-thread blocks otherwise. */
 int hh_context_set_dst_algorithm(hh_context *ctx, int algo);
