@@ -853,12 +853,23 @@ extern "C" int hh_plan_set_tile_width(hh_plan *p, int tw) {
     return HH_OK;
 }
 
+// Built-in shape of FP64 elimination on FP32 storage (mixed precision), on
+// top of the FP64 one: in 3D from n = 255, 16 rows per thread (255^3: W = 16,
+// 114 against 137 us per launch for 8 rows).
+static void fp32_storage_rows(const hh_plan *p, int n, long nmodes, hh::TriHelm &t) {
+    if (p->dim == 3 && t.mmax == 8 && n >= 255) {
+        t.mmax = 16;
+        hh::tri_shape_rows(n, nmodes, 16, &t.W, &t.P);
+    }
+}
+
 extern "C" int hh_plan_get_tri_shape(const hh_plan *p, int *W, int *P, int *rows_per_thread, int *rhs_per_thread) {
     if (!p) return fail(HH_ERR_ARG, "null plan");
     hh::TriHelm t;
     int n = p->dim == 2 ? p->Ny : p->Nz;
     long nmodes = p->dim == 2 ? p->Nx : (long)p->Nx * p->Ny;
     hh::tri_shape(n, nmodes, hh::F64, &t.W, &t.P, &t.mmax);
+    if (p->prec == HH_PREC_MIXED) fp32_storage_rows(p, n, nmodes, t);
     t.nr = 1;
     if (p->tri_W > 0) {
         t.W = p->tri_W;
@@ -905,6 +916,7 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
     hh::tri_shape(n, nmodes, tri_comp == 0 && tc == 0 ? hh::F64 : hh::F32, &t.W, &t.P, &t.mmax);
     if (tri_comp == 0 && tc == 1) {   // FP64 elimination on FP32 storage: FP64 register budget
         hh::tri_shape(n, nmodes, hh::F64, &t.W, &t.P, &t.mmax);
+        fp32_storage_rows(p, n, nmodes, t);
     }
     t.nr = 1;
     // Real bands, FP64 elimination, 16 rows per thread: two right-hand sides

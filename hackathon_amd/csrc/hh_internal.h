@@ -79,6 +79,8 @@ hipError_t launch_compact(const Compact &cd, hipStream_t st);
 
 // Choose a tridiagonal block shape for a system of length n and nmodes modes.
 void tri_shape(int n, long nmodes, int prec, int *W, int *P, int *mmax);
+// block width W and partitions P for mm rows per thread (256-thread blocks at most)
+void tri_shape_rows(int n, long nmodes, int mm, int *W, int *P);
 
 // Compact-stencil right-hand-side assembly (2D_Program/main_functions/rhs.c,
 // 3D_Program/main_functions/rhs.c and the partial derivatives).

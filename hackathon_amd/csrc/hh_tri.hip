@@ -509,6 +509,11 @@ void tri_shape(int n, long nmodes, int prec, int *W, int *P, int *mmax) {
     int mm = n > 512 ? 16 : 8;
     if (n <= 16) mm = 4;
     (void)prec;
+    tri_shape_rows(n, nmodes, mm, W, P);
+    *mmax = mm;
+}
+
+void tri_shape_rows(int n, long nmodes, int mm, int *W, int *P) {
     int p = (n + mm - 1) / mm;
     if (p > n / 2) p = n / 2;
     if (p < 1) p = 1;
@@ -519,7 +524,7 @@ void tri_shape(int n, long nmodes, int prec, int *W, int *P, int *mmax) {
     while (pw * 2 <= w) pw *= 2;
     w = pw;
     while (w > 1 && w / 2 >= nmodes) w /= 2;
-    *W = w; *P = p; *mmax = mm;
+    *W = w; *P = p;
 }
 
 template <typename T, typename Ti, typename To, int MM, int DIM, bool REAL>

@@ -168,8 +168,9 @@ int hh_plan_set_operators(hh_plan *plan, int nop, const double *coef);
  * speed only: every right-hand side gives the same bits for a given W and
  * rows_per_thread, whatever else is in its batch and whatever rhs_per_thread.
  * get_tri_shape reports the shape a solve uses; for the built-in heuristic
- * it reports rhs_per_thread = 1, though a solve with real bands in FP64 at
- * 16 rows per thread takes 2 when its batch gives 2048 blocks or more. */
+ * it reports rhs_per_thread = 1, though a solve with real bands at 16 rows
+ * per thread takes 2 when its batch gives 2048 blocks or more (in mixed
+ * precision, when an operator also has two or more right-hand sides). */
 int hh_plan_set_tri_shape(hh_plan *plan, int W, int rows_per_thread, int rhs_per_thread);
 int hh_plan_get_tri_shape(const hh_plan *plan, int *W, int *P, int *rows_per_thread, int *rhs_per_thread);
 /* 2D FFT path: layout of the spectrum between the row DSTs and the mode

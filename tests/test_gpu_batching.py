@@ -92,6 +92,32 @@ def test_fixed_sweeps_3d_mixed(H, ctx, oracle):
     assert err <= 1e-10
 
 
+def test_mixed_3d_255_row_shapes(H, ctx):
+    """3D 255^3 in mixed precision: the built-in shape (16 rows per thread
+    for FP32 storage) and the FP64 one (8 rows) both land within 1e-10 of
+    the FP64 solve of the same right-hand side (the FP64 solver is pinned to
+    the oracle by the parity tests)."""
+    import ctypes
+    import torch
+    N = 255
+    h = 1.0 / (N + 1)
+    coef = H.coefficients3d(6, N, N, N, h, h, h, np.full(N + 2, 900.0 + 0j))
+    g = torch.Generator(device="cuda").manual_seed(21)
+    rhs = torch.randn((1, N, N, N), dtype=torch.complex128, device="cuda", generator=g)
+    ref = H.Plan3D(ctx, N, N, N, coef).solve(rhs).cpu().numpy().ravel()
+    plan = H.Plan3D(ctx, N, N, N, coef, precision=H.PREC_MIXED)
+    plan.set_refine(0.0, 2)
+    v = [ctypes.c_int() for _ in range(4)]
+    H._check(H.lib().hh_plan_get_tri_shape(plan.h, *[ctypes.byref(x) for x in v]))
+    assert [x.value for x in v] == [16, 16, 16, 1]
+    for shape in ((0, 0, 0), (8, 8, 1)):
+        H._check(H.lib().hh_plan_set_tri_shape(plan.h, *shape))
+        sol = plan.solve(rhs).cpu().numpy().ravel()
+        err = relerr(sol, ref)
+        say(f"3D 255^3 mixed shape={shape}: rel err vs FP64 {err:.2e}")
+        assert err <= 1e-10
+
+
 def test_set_operators_matches_new_plan(H, ctx, oracle):
     N = 127
     probs = [oracle.solve2d(o, N, 1, [k, 0.0, 3, 2, 0.3], threads=8) for o, k in
