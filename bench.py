@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--lanes", type=int, default=4, help="serve: server lanes (streams)")
     ap.add_argument("--problems", type=int, default=512, help="uneven: number of problems")
     ap.add_argument("--sweeps", type=int, default=2, help="uneven: fixed refinement sweeps")
+    ap.add_argument("--tune", action="store_true",
+                    help="uneven: autotune each group's tridiagonal shape before capture (eager timing; "
+                         "the built-in shapes measured faster in the graph)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB three times with an elementwise kernel (y = x * 1), the "
                          "known-size reference for scaling HBM counters")
@@ -304,7 +307,16 @@ def run_uneven(args, torch, H):
         del m, f_e
         solves64.append((p64, F, torch.empty_like(F)))
         solvesmx.append((pmx, F, torch.empty_like(F)))
-        info.append({"N": N, "order": order, "problems": B, "k": ks, "ops": ops, "u": u_ex})
+        shapes = {}
+        if args.tune:
+            # the autotuner picks each group's tridiagonal shape (row-major spectra), before capture
+            from hackathon_amd import autotune
+            t_tune = time.perf_counter()
+            for name, plan, buf in (("fp64", p64, solves64[-1][2]), ("mixed", pmx, solvesmx[-1][2])):
+                best = autotune.tune(plan, F, buf, reps=2, tiles=(-1,))
+                shapes[name] = {"W": best["W"], "rows": best["rows"], "rhs": best["rhs"], "us": round(best["us"], 1)}
+            log(f"uneven: tuned N={N} order={order} in {time.perf_counter() - t_tune:.1f} s: {shapes}")
+        info.append({"N": N, "order": order, "problems": B, "k": ks, "ops": ops, "u": u_ex, "shapes": shapes})
         total_dof += B * N * N
         log(f"uneven: group N={N} order={order}: {B} problems")
     torch.cuda.synchronize()
@@ -356,7 +368,9 @@ def run_uneven(args, torch, H):
                      out[best]["ms_per_launch"],
                      {"workload": f"{P} uneven 2D problems, N log-uniform in {{63..2047}}, k~U[1,80], order~U{{4,6}}, "
                                   f"manufactured sines (seed {args.seed}); one HIP graph launch per step",
-                      "problems": P, "total_dof": total_dof, "groups": sizes},
+                      "problems": P, "total_dof": total_dof, "groups": sizes,
+                      "tridiagonal_shapes": "built-in" if not args.tune else
+                      {f"N{gi['N']}_order{gi['order']}": gi["shapes"] for gi in info}},
                      dtype="f32+f64" if best == "mixed" else "f64", value_path=best,
                      mixed_over_fp64=out["mixed"]["dof_per_s"] / out["fp64"]["dof_per_s"], results=out,
                      oracle_check=check)

@@ -66,14 +66,16 @@ def time_tri(plan, rhs, out, reps=3):
     return 1e3 * sum(ms) / reps, [ms[k] / max(nl[k], 1) for k in range(5)]
 
 
-def tune(plan, rhs, out, candidates=CANDIDATES, reps=3, verbose=False):
-    """Try every shape (and, in 2D, spectrum layout) that fits; leave the
+def tune(plan, rhs, out, candidates=CANDIDATES, reps=3, verbose=False, tiles=None):
+    """Try the built-in shape and every candidate shape (and, in 2D,
+    spectrum layout, or only the given tile widths) that fits; leave the
     fastest, by kernel time of the whole solve, set on the plan."""
     lib = H.lib()
     results = []
     batch = plan._batch_of(rhs)
-    tiles = (0, -1, 8, 16, 32) if plan.dim == 2 else (-1,)
-    for w, m, r in candidates:
+    if tiles is None:
+        tiles = (0, -1, 8, 16, 32) if plan.dim == 2 else (-1,)
+    for w, m, r in [(0, 0, 1)] + list(candidates):
         if r > 1 and batch < 2 * r:
             continue
         if lib.hh_plan_set_tri_shape(plan.h, w, m, r) != 0:
