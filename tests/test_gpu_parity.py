@@ -1,7 +1,7 @@
 """GPU parity: every kernel set against the CPU FP64 oracle on the same inputs.
 
-Tolerances (relative, max norm): FP64 paths <= 1e-12 against the oracle
-(north-star bar); the DST alone <= 1e-13.  Mixed precision <= 1e-10.
+Tolerances (relative, max norm): FP64 paths <= 1e-12 against the oracle;
+the DST alone <= 1e-13.  Mixed precision <= 1e-10.
 """
 import os
 import sys
