@@ -28,7 +28,7 @@ ARCH = os.environ.get("HH_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = ["-O3", f"--offload-arch={ARCH}", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-result",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", f"-I{INCLUDE}"]
 
-CORE_SOURCES = ["hh_dst.hip", "hh_tri.hip", "hh_stencil.hip", "hh_api.hip", "hh_host.hip", "hh_serve.hip"]
+CORE_SOURCES = ["hh_dst.hip", "hh_tri.hip", "hh_stencil.hip", "hh_api.hip", "hh_host.hip", "hh_serve.hip", "hh_modes.hip"]
 LIB = os.path.join(HERE, "libhh.so")
 REF2D = os.path.join(HERE, "libhh_ref2d.so")
 REF3D = os.path.join(HERE, "libhh_ref3d.so")

@@ -116,6 +116,20 @@ struct hh_plan {
     int tile_w = 0;
     int last_sweeps = 0;
     double last_rel = 0;
+    // eigenvalues 2cos(k pi/(N+1)) (host copies of lamx / lamy)
+    std::vector<double> hlamx, hlamy;
+    // near-resonant modes of the current operators and their reference-order
+    // factors (hh_modes.hip), with their device copies
+    double kappa_flag = hh::HH_KAPPA_FLAG;
+    hh::ModeSet ms;
+    int *d_foff = nullptr, *d_fmode = nullptr;
+    uint32_t *d_bits = nullptr;
+    void *d_facc = nullptr, *d_facr = nullptr;
+    size_t cap_foff = 0, cap_fmode = 0, cap_bits = 0, cap_facc = 0, cap_facr = 0;
+    // Generation: bumped whenever a change would invalidate a recorded graph
+    // (device tables reallocated, band kind, operator count or launch shapes
+    // changed); graphs record it at capture and refuse to replay a newer plan.
+    long gen = 0;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -645,24 +659,76 @@ static void fill_real(hh_plan *p) {
     p->real_op = real;
 }
 
-static void fill_table(hh_plan *p, int nop, const double *coef) {
-    if (p->dim == 2) {
-        const int Ny = p->Ny;
+static void build_table(int dim, int n, int nop, const double *coef, std::vector<cd> &tab) {
+    if (dim == 2) {
         static const int src[6] = {4, 5, 2, 3, 0, 1};
-        p->hcoef.resize(6 * (size_t)Ny * nop);
+        tab.resize(6 * (size_t)n * nop);
         for (int o = 0; o < nop; o++)
-            for (int q = 0; q < Ny; q++)
+            for (int q = 0; q < n; q++)
                 for (int t = 0; t < 6; t++)
-                    p->hcoef[((size_t)o * Ny + q) * 6 + t] = ld(coef, ((size_t)o * 6 + src[t]) * Ny + q);
+                    tab[((size_t)o * n + q) * 6 + t] = ld(coef, ((size_t)o * 6 + src[t]) * n + q);
     } else {
-        const int Nz = p->Nz;
-        p->hcoef.resize(12 * (size_t)Nz * nop);
+        tab.resize(12 * (size_t)n * nop);
         for (int o = 0; o < nop; o++)
-            for (int l = 0; l < Nz; l++)
+            for (int l = 0; l < n; l++)
                 for (int q = 0; q < 12; q++)
-                    p->hcoef[((size_t)o * Nz + l) * 12 + q] = ld(coef, ((size_t)o * 12 + q) * Nz + l);
+                    tab[((size_t)o * n + l) * 12 + q] = ld(coef, ((size_t)o * 12 + q) * n + l);
     }
+}
+
+static void fill_table(hh_plan *p, int nop, const double *coef) {
+    build_table(p->dim, p->dim == 2 ? p->Ny : p->Nz, nop, coef, p->hcoef);
     fill_real(p);
+    hh::classify_modes(p->dim, p->Nx, p->Ny, p->Nz, nop, p->hcoef.data(), p->hlamx.data(), p->hlamy.data(),
+                       p->real_op != 0, p->kappa_flag, p->ms);
+}
+
+// eigenvalues 2cos(k pi/(N+1)), k = 1..N (LU.c:12-14)
+static void eigenvalues(int N, std::vector<double> &v) {
+    v.resize(N);
+    for (int k = 0; k < N; k++) v[k] = 2.0 * std::cos((k + 1) * M_PI / (N + 1));
+}
+
+// Device array holding at least `bytes`; a new allocation bumps the plan's
+// generation (recorded graphs hold the old address).
+static int ensure_dev(hh_plan *p, void **ptr, size_t *cap, size_t bytes) {
+    if (*ptr && bytes <= *cap) return HH_OK;
+    hh_context *c = p->ctx;
+    if (*ptr) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        dev_free(c, *ptr, *cap);
+        *ptr = nullptr;
+        *cap = 0;
+    }
+    const size_t b = bytes ? bytes : 16;
+    int rc = dev_alloc(c, ptr, b);
+    if (rc) return rc;
+    *cap = b;
+    p->gen++;
+    return HH_OK;
+}
+
+// Copies the near-resonant mode tables to the device, stream-ordered after
+// the work already queued (the host vectors stay untouched until the next
+// fill_table, which waits for the stream first).
+static int upload_modes(hh_plan *p) {
+    hh_context *c = p->ctx;
+    const hh::ModeSet &m = p->ms;
+    int rc;
+    if ((rc = ensure_dev(p, (void **)&p->d_foff, &p->cap_foff, m.foff.size() * sizeof(int)))) return rc;
+    if ((rc = ensure_dev(p, (void **)&p->d_fmode, &p->cap_fmode, m.fmode.size() * sizeof(int)))) return rc;
+    if ((rc = ensure_dev(p, (void **)&p->d_bits, &p->cap_bits, m.bits.size() * sizeof(uint32_t)))) return rc;
+    if ((rc = ensure_dev(p, &p->d_facc, &p->cap_facc, m.facc.size() * sizeof(cd)))) return rc;
+    if ((rc = ensure_dev(p, &p->d_facr, &p->cap_facr, m.facr.size() * sizeof(double)))) return rc;
+    auto cp = [&](void *d, const void *h, size_t bytes) -> int {
+        if (bytes) HIPCHK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream));
+        return HH_OK;
+    };
+    if ((rc = cp(p->d_foff, m.foff.data(), m.foff.size() * sizeof(int)))) return rc;
+    if ((rc = cp(p->d_fmode, m.fmode.data(), m.fmode.size() * sizeof(int)))) return rc;
+    if ((rc = cp(p->d_bits, m.bits.data(), m.bits.size() * sizeof(uint32_t)))) return rc;
+    if ((rc = cp(p->d_facc, m.facc.data(), m.facc.size() * sizeof(cd)))) return rc;
+    return cp(p->d_facr, m.facr.data(), m.facr.size() * sizeof(double));
 }
 
 static int upload_plan(hh_plan *p, int ncoef, int n) {
@@ -683,22 +749,65 @@ static int upload_plan(hh_plan *p, int ncoef, int n) {
     if ((rc = dev_alloc(c, &p->coefr32, b32 / 2))) return rc;
     HIPCHK(hipMemcpy(p->coefr64, p->hr64.data(), b64 / 2, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(p->coefr32, p->hr32.data(), b32 / 2, hipMemcpyHostToDevice));
-    auto eig = [](int N, std::vector<double> &v) {
-        v.resize(N);
-        for (int k = 0; k < N; k++) v[k] = 2.0 * std::cos((k + 1) * M_PI / (N + 1));   // LU.c:12-14
-    };
-    std::vector<double> lx, ly;
-    eig(p->Nx, lx);
+    const std::vector<double> &lx = p->hlamx, &ly = p->hlamy;
     rc = dev_alloc(c, (void **)&p->lamx, lx.size() * 8);
     if (rc) return rc;
     HIPCHK(hipMemcpy(p->lamx, lx.data(), lx.size() * 8, hipMemcpyHostToDevice));
     if (p->dim == 3) {
-        eig(p->Ny, ly);
         rc = dev_alloc(c, (void **)&p->lamy, ly.size() * 8);
         if (rc) return rc;
         HIPCHK(hipMemcpy(p->lamy, ly.data(), ly.size() * 8, hipMemcpyHostToDevice));
     }
+    if ((rc = upload_modes(p))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
     return HH_OK;
+}
+
+// ------------------------------------------------------------------ host-only mode classification
+static bool table_real(const std::vector<cd> &tab) {
+    for (const cd &v : tab)
+        if (v.imag() != 0.0) return false;
+    return true;
+}
+
+extern "C" int hh_operator_modes(int dim, int Nx, int Ny, int Nz, const double *coef, double kappa, int *nflagged,
+                                 int *modes, int cap, int *singular, double *kappa_worst) {
+    if ((dim != 2 && dim != 3) || Nx < 2 || Ny < 2 || (dim == 3 && Nz < 2) || !coef || !(kappa >= 0) ||
+        (cap > 0 && !modes))
+        return fail(HH_ERR_ARG, "hh_operator_modes: bad arguments");
+    std::vector<cd> tab;
+    build_table(dim, dim == 2 ? Ny : Nz, 1, coef, tab);
+    std::vector<double> lx, ly;
+    eigenvalues(Nx, lx);
+    if (dim == 3) eigenvalues(Ny, ly);
+    hh::ModeSet ms;
+    hh::classify_modes(dim, Nx, Ny, Nz, 1, tab.data(), lx.data(), ly.data(), table_real(tab), kappa, ms);
+    if (nflagged) *nflagged = (int)ms.fmode.size();
+    for (int i = 0; i < (int)ms.fmode.size() && i < cap; i++) modes[i] = ms.fmode[i];
+    if (singular) *singular = ms.singular[0];
+    if (kappa_worst) *kappa_worst = ms.kappa_worst[0];
+    return HH_OK;
+}
+
+extern "C" int hh_operator_mode_factors(int dim, int Nx, int Ny, int Nz, const double *coef, int mode,
+                                        double *factors) {
+    const long nmodes = dim == 2 ? Nx : (long)Nx * Ny;
+    if ((dim != 2 && dim != 3) || Nx < 2 || Ny < 2 || (dim == 3 && Nz < 2) || !coef || !factors || mode < 0 ||
+        mode >= nmodes)
+        return fail(HH_ERR_ARG, "hh_operator_mode_factors: bad arguments");
+    const int n = dim == 2 ? Ny : Nz;
+    std::vector<cd> tab;
+    build_table(dim, n, 1, coef, tab);
+    std::vector<double> lx, ly;
+    eigenvalues(Nx, lx);
+    if (dim == 3) eigenvalues(Ny, ly);
+    // classify with kappa = 0: every mode flagged, entry index = mode index
+    hh::ModeSet ms;
+    const bool real = table_real(tab);
+    hh::classify_modes(dim, Nx, Ny, Nz, 1, tab.data(), lx.data(), ly.data(), real, 0.0, ms);
+    const cd *f = ms.facc.data() + (size_t)mode * n * 3;
+    for (long i = 0; i < 3L * n; i++) { factors[2 * i] = f[i].real(); factors[2 * i + 1] = f[i].imag(); }
+    return ms.singular[0] ? fail(HH_ERR_SINGULAR, "operator is singular to working precision") : HH_OK;
 }
 
 static int plan_check_dims(int dim, int Nx, int Ny, int Nz) {
@@ -733,6 +842,7 @@ extern "C" int hh_plan2d_create_multi(hh_context *ctx, int Nx, int Ny, int nop, 
     hh_plan *p = new hh_plan();
     p->ctx = ctx; p->dim = 2; p->Nx = Nx; p->Ny = Ny; p->Nz = 1; p->prec = precision; p->nop = nop;
     p->cap_nop = nop;
+    eigenvalues(Nx, p->hlamx);
     fill_table(p, nop, coef);
     if ((rc = upload_plan(p, 6, Ny))) { hh_plan_destroy(p); return rc; }
     *out = p;
@@ -761,6 +871,8 @@ extern "C" int hh_plan3d_create_multi(hh_context *ctx, int Nx, int Ny, int Nz, i
     hh_plan *p = new hh_plan();
     p->ctx = ctx; p->dim = 3; p->Nx = Nx; p->Ny = Ny; p->Nz = Nz; p->prec = precision; p->nop = nop;
     p->cap_nop = nop;
+    eigenvalues(Nx, p->hlamx);
+    eigenvalues(Ny, p->hlamy);
     fill_table(p, nop, coef);
     if ((rc = upload_plan(p, 12, Nz))) { hh_plan_destroy(p); return rc; }
     *out = p;
@@ -780,6 +892,11 @@ extern "C" int hh_plan_destroy(hh_plan *p) {
     dev_free(c, p->coefr32, (size_t)nc * n * p->cap_nop * 4);
     dev_free(c, p->lamx, (size_t)p->Nx * 8);
     if (p->lamy) dev_free(c, p->lamy, (size_t)p->Ny * 8);
+    dev_free(c, p->d_foff, p->cap_foff);
+    dev_free(c, p->d_fmode, p->cap_fmode);
+    dev_free(c, p->d_bits, p->cap_bits);
+    dev_free(c, p->d_facc, p->cap_facc);
+    dev_free(c, p->d_facr, p->cap_facr);
     delete p;
     return HH_OK;
 }
@@ -808,6 +925,7 @@ extern "C" size_t hh_plan_workspace_bytes(const hh_plan *p, int batch) {
 
 extern "C" int hh_plan_set_refine(hh_plan *p, double tol, int max_iter) {
     if (!p || tol < 0 || max_iter < 0) return fail(HH_ERR_ARG, "hh_plan_set_refine: bad arguments");
+    p->gen++;
     p->tol = tol;
     p->max_iter = max_iter;
     return HH_OK;
@@ -815,6 +933,7 @@ extern "C" int hh_plan_set_refine(hh_plan *p, double tol, int max_iter) {
 
 extern "C" int hh_plan_set_tri_shape(hh_plan *p, int W, int rows_per_thread, int rhs_per_thread) {
     if (!p) return fail(HH_ERR_ARG, "null plan");
+    p->gen++;
     if (W == 0 && rows_per_thread == 0) { p->tri_W = p->tri_mm = 0; p->tri_nr = 1; return HH_OK; }
     int n = p->dim == 2 ? p->Ny : p->Nz;
     if (W < 1 || W > 64 || (rows_per_thread != 4 && rows_per_thread != 8 && rows_per_thread != 16) ||
@@ -838,6 +957,7 @@ extern "C" int hh_plan_set_tri_shape(hh_plan *p, int W, int rows_per_thread, int
 
 extern "C" int hh_plan_set_real_operator(hh_plan *p, int mode) {
     if (!p) return fail(HH_ERR_ARG, "null plan");
+    p->gen++;
     if (mode < -1 || mode > 0) return fail(HH_ERR_ARG, "real-operator mode must be -1 (automatic) or 0 (complex bands)");
     p->real_mode = mode;
     return HH_OK;
@@ -847,6 +967,7 @@ extern "C" int hh_plan_is_real_operator(const hh_plan *p) { return p && p->real_
 
 extern "C" int hh_plan_set_tile_width(hh_plan *p, int tw) {
     if (!p) return fail(HH_ERR_ARG, "null plan");
+    p->gen++;
     if (tw != -1 && (tw < 0 || tw > 64 || (tw & (tw - 1))))
         return fail(HH_ERR_ARG, "tile width must be -1, 0 or a power of two up to 64 (got %d)", tw);
     p->tile_w = tw;
@@ -883,6 +1004,36 @@ extern "C" int hh_plan_get_tri_shape(const hh_plan *p, int *W, int *P, int *rows
     if (rows_per_thread) *rows_per_thread = t.mmax;
     if (rhs_per_thread) *rhs_per_thread = t.nr;
     return HH_OK;
+}
+
+extern "C" int hh_plan_set_resonance_threshold(hh_plan *p, double kappa) {
+    if (!p || !(kappa >= 0)) return fail(HH_ERR_ARG, "hh_plan_set_resonance_threshold: bad arguments");
+    hh_context *c = p->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = set_device(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int old_nfx = p->ms.nfx;
+    p->kappa_flag = kappa;
+    hh::classify_modes(p->dim, p->Nx, p->Ny, p->Nz, p->nop, p->hcoef.data(), p->hlamx.data(), p->hlamy.data(),
+                       p->real_op != 0, p->kappa_flag, p->ms);
+    if (p->ms.nfx != old_nfx) p->gen++;
+    return upload_modes(p);
+}
+
+extern "C" int hh_plan_mode_info(const hh_plan *p, int op, int *flagged, int *singular, double *kappa) {
+    if (!p || op < 0 || op >= p->nop) return fail(HH_ERR_ARG, "hh_plan_mode_info: bad arguments");
+    if (flagged) *flagged = p->ms.foff[op + 1] - p->ms.foff[op];
+    if (singular) *singular = p->ms.singular[op];
+    if (kappa) *kappa = p->ms.kappa_worst[op];
+    return HH_OK;
+}
+
+extern "C" int hh_plan_flagged_modes(const hh_plan *p, int op, int *modes, int cap) {
+    if (!p || op < 0 || op >= p->nop || (cap > 0 && !modes)) return fail(HH_ERR_ARG, "hh_plan_flagged_modes: bad arguments");
+    const int n = p->ms.foff[op + 1] - p->ms.foff[op];
+    for (int i = 0; i < n && i < cap; i++) modes[i] = p->ms.fmode[p->ms.foff[op] + i];
+    return n;
 }
 
 extern "C" int hh_plan_last_refine(const hh_plan *p, int *sweeps, double *rel) {
@@ -937,9 +1088,25 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
         t.P = (n + t.mmax - 1) / t.mmax;
         if (t.P > n / 2) t.P = n / 2;
     }
+    // near-resonant modes: reference-order substitution from the host's
+    // factors (FP64 arithmetic), then the partitioned kernel for the rest
+    t.nfx = p->ms.nfx;
+    if (t.nfx > 0) {
+        t.fbits = p->d_bits;
+        t.fwords = p->ms.words;
+        t.foff = p->d_foff;
+        t.fmode = p->d_fmode;
+        t.fac = t.real ? p->d_facr : p->d_facc;
+    }
+    // The two kernels touch disjoint outputs (the partitioned kernel reads the
+    // near-resonant modes' rows but does not store them).  Run concurrently on
+    // a second stream they took as long together (C2: 651 us) as one after
+    // the other: the partitioned kernel's blocks fill the chip.
     auto tri = [&](void *buf) -> int {
         TimeScope ts(c, 2, (double)t.field * batch * (tc ? 16 : 32));
-        hipError_t e = hh::launch_tri_helm(t, hh::DstTypes{tc, tri_comp, tc}, buf, buf, c->stream);
+        hipError_t e = hh::launch_tri_seq(t, hh::DstTypes{tc, 0, tc}, buf, buf, c->stream);
+        if (e != hipSuccess) return fail(HH_ERR_DEVICE, "near-resonant mode launch failed: %s", hipGetErrorString(e));
+        e = hh::launch_tri_helm(t, hh::DstTypes{tc, tri_comp, tc}, buf, buf, c->stream);
         if (e != hipSuccess) return fail(HH_ERR_DEVICE, "tridiagonal launch failed: %s", hipGetErrorString(e));
         return HH_OK;
     };
@@ -1012,32 +1179,44 @@ static int norms_sync(hh_context *c, long n, const void *x, int x_f32, const voi
     return HH_OK;
 }
 
+// HH_ERR_SINGULAR when an operator that right-hand sides 0..batch-1 use is
+// singular to working precision.
+static int check_singular(const hh_plan *p, int batch) {
+    const int used = batch < p->nop ? batch : p->nop;
+    for (int o = 0; o < used; o++)
+        if (p->ms.singular[o])
+            return fail(HH_ERR_SINGULAR, "operator %d is singular to working precision (a mode's condition "
+                                         "estimate %.3g exceeds %.0e): k^2 is at or next to an eigenvalue",
+                        o, p->ms.kappa_worst[o], hh::HH_SINGULAR_KAPPA);
+    return HH_OK;
+}
+
 // hh_solve with the context lock held.
-static int solve_locked(hh_plan *p, const void *rhs, void *sol, int batch) {
+// A solve on workspace `scratch` (hh_plan_workspace_bytes(p, batch) bytes:
+// the context's scratch, a graph's own workspace or the caller's).
+static int solve_ws(hh_plan *p, const void *rhs, void *sol, int batch, void *scratch) {
     hh_context *c = p->ctx;
-    int rc = set_device(c);
-    if (rc) return rc;
-    size_t ws_bytes = hh_plan_workspace_bytes(p, batch);
-    if (ws_bytes && (rc = ensure_scratch(c, ws_bytes))) return rc;
+    int rc;
     const long n = hh_plan_field_size(p) * batch;
     if (p->prec == HH_PREC_F64) {
-        void *ws = (plan_all_fft(p) && (p->dim == 3 || p->tile_w == 0)) ? nullptr : c->scratch;
+        void *ws = (plan_all_fft(p) && (p->dim == 3 || p->tile_w == 0)) ? nullptr : scratch;
         return solve_core(p, 0, 0, 0, 0, rhs, sol, ws, batch);
     }
-    // mixed: FP32 transforms and storage, FP64 elimination, FP64 residual
-    // and update; the correction lives in FP32 scratch, followed (2D) by the
+    // mixed: FP32 transforms, storage and elimination of the well-separated
+    // modes, FP64 substitution of the near-resonant ones, FP64 residual and
+    // update; the correction lives in FP32 scratch, followed (2D) by the
     // FP32 mode-tiled spectrum of the solves
-    void *dlt = c->scratch;
-    void *tws = (p->dim == 2 && p->tile_w != 0) ? (void *)((char *)c->scratch + (size_t)n * 8) : dlt;
+    void *dlt = scratch;
+    void *tws = (p->dim == 2 && p->tile_w != 0) ? (void *)((char *)scratch + (size_t)n * 8) : dlt;
     if (!plan_all_fft(p)) return fail(HH_ERR_UNSUPPORTED, "mixed precision needs power-of-two N+1");
     if (rhs == sol) return fail(HH_ERR_ARG, "mixed precision solve needs d_sol != d_rhs");
-    if ((rc = solve_core(p, 0, 1, 0, 0, rhs, sol, tws, batch))) return rc;   // u0 = A32^{-1} F
+    if ((rc = solve_core(p, 0, 1, 0, 1, rhs, sol, tws, batch))) return rc;   // u0 = A32^{-1} F
     if (p->tol == 0) {
         // fixed number of sweeps: no norms, no host synchronisation, so the
         // whole solve can be recorded into a HIP graph
         for (int it = 0; it < p->max_iter; it++) {
             if ((rc = residual_into(p, sol, rhs, dlt, 1, batch))) return rc;
-            if ((rc = solve_core(p, 1, 1, 1, 0, dlt, dlt, tws, batch))) return rc;
+            if ((rc = solve_core(p, 1, 1, 1, 1, dlt, dlt, tws, batch))) return rc;
             TimeScope ts(c, 3, (double)n * 40);
             hipError_t e = hh::launch_axpy_update(n, sol, dlt, 1, -1.0, c->stream);
             if (e != hipSuccess) return fail(HH_ERR_DEVICE, "update launch failed: %s", hipGetErrorString(e));
@@ -1058,7 +1237,7 @@ static int solve_locked(hh_plan *p, const void *rhs, void *sol, int batch) {
         if ((rc = norms_sync(c, n, dlt, 1, nullptr, nr))) return rc;
         rel = nr[0] / fn;
         if (rel <= p->tol || sweeps >= p->max_iter) break;
-        if ((rc = solve_core(p, 1, 1, 1, 0, dlt, dlt, tws, batch))) return rc;   // A d = r
+        if ((rc = solve_core(p, 1, 1, 1, 1, dlt, dlt, tws, batch))) return rc;   // A d = r
         TimeScope ts(c, 3, (double)n * 40);
         hipError_t e = hh::launch_axpy_update(n, sol, dlt, 1, -1.0, c->stream);    // u -= d
         if (e != hipSuccess) return fail(HH_ERR_DEVICE, "update launch failed: %s", hipGetErrorString(e));
@@ -1067,6 +1246,16 @@ static int solve_locked(hh_plan *p, const void *rhs, void *sol, int batch) {
     p->last_sweeps = sweeps;
     p->last_rel = rel;
     return HH_OK;
+}
+
+static int solve_locked(hh_plan *p, const void *rhs, void *sol, int batch) {
+    hh_context *c = p->ctx;
+    int rc = check_singular(p, batch);
+    if (rc) return rc;
+    if ((rc = set_device(c))) return rc;
+    size_t ws_bytes = hh_plan_workspace_bytes(p, batch);
+    if (ws_bytes && (rc = ensure_scratch(c, ws_bytes))) return rc;
+    return solve_ws(p, rhs, sol, batch, c->scratch);
 }
 
 extern "C" int hh_solve(hh_plan *p, const void *rhs, void *sol, int batch) {
@@ -1082,6 +1271,13 @@ struct hh_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int nodes = 0;
+    // the graph's own workspace: later solves on the context may grow (and
+    // move) the context's scratch, never this
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    // plans and their generations at capture: a plan changed since (tables
+    // reallocated, band kind, operator count or shapes) refuses the replay
+    std::vector<std::pair<hh_plan *, long>> plans;
 };
 
 // Builds every table a plan's solves read (twiddles or DST matrices) so
@@ -1117,12 +1313,18 @@ extern "C" int hh_graph_create(hh_context *c, int nsolves, hh_plan *const *plans
         if (p->prec == HH_PREC_MIXED && p->tol != 0)
             return fail(HH_ERR_UNSUPPORTED, "hh_graph_create: mixed-precision plan %d needs fixed sweeps "
                                             "(hh_plan_set_refine(plan, 0, sweeps))", i);
+        if ((rc = check_singular(p, batch[i]))) return rc;
         if ((rc = prepare_tables(p))) return rc;
         size_t b = hh_plan_workspace_bytes(p, batch[i]);
         if (b > ws) ws = b;
     }
-    if (ws && (rc = ensure_scratch(c, ws))) return rc;
-    HIPCHK(hipStreamSynchronize(c->stream));
+    void *gws = nullptr;
+    if (ws && (rc = dev_alloc(c, &gws, ws))) return rc;
+    auto drop_ws = [&]() { if (gws) dev_free(c, gws, ws); };
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        drop_ws();
+        return fail(HH_ERR_DEVICE, "hh_graph_create: stream synchronisation failed");
+    }
     hipStream_t cap = nullptr;
     HIPCHK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
     hipStream_t saved = c->stream;
@@ -1132,7 +1334,7 @@ extern "C" int hh_graph_create(hh_context *c, int nsolves, hh_plan *const *plans
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        for (int i = 0; i < nsolves && rc == HH_OK; i++) rc = solve_locked(plans[i], d_rhs[i], d_sol[i], batch[i]);
+        for (int i = 0; i < nsolves && rc == HH_OK; i++) rc = solve_ws(plans[i], d_rhs[i], d_sol[i], batch[i], gws);
         hipError_t e2 = hipStreamEndCapture(cap, &g);
         if (rc == HH_OK && e2 != hipSuccess) rc = fail(HH_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e2));
     } else {
@@ -1143,18 +1345,23 @@ extern "C" int hh_graph_create(hh_context *c, int nsolves, hh_plan *const *plans
     (void)hipStreamDestroy(cap);
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
+        drop_ws();
         return rc;
     }
     hipGraphExec_t x = nullptr;
     e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
         (void)hipGraphDestroy(g);
+        drop_ws();
         return fail(HH_ERR_DEVICE, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
     }
     hh_graph *G = new hh_graph();
     G->ctx = c;
     G->graph = g;
     G->exec = x;
+    G->ws = gws;
+    G->ws_bytes = ws;
+    for (int i = 0; i < nsolves; i++) G->plans.emplace_back(plans[i], plans[i]->gen);
     size_t nn = 0;
     if (hipGraphGetNodes(g, nullptr, &nn) == hipSuccess) G->nodes = (int)nn;
     *out = G;
@@ -1167,6 +1374,10 @@ extern "C" int hh_graph_launch(hh_graph *G) {
     std::lock_guard<std::mutex> lk(c->mu);
     int rc = set_device(c);
     if (rc) return rc;
+    for (const auto &pg : G->plans)
+        if (pg.first->gen != pg.second)
+            return fail(HH_ERR_ARG, "hh_graph_launch: a plan of the graph changed since it was recorded "
+                                    "(tables reallocated, band kind, operator count or shapes); record it again");
     HIPCHK(hipGraphLaunch(G->exec, c->stream));
     return HH_OK;
 }
@@ -1178,6 +1389,7 @@ extern "C" int hh_graph_destroy(hh_graph *G) {
     (void)hipStreamSynchronize(G->ctx->stream);
     if (G->exec) (void)hipGraphExecDestroy(G->exec);
     if (G->graph) (void)hipGraphDestroy(G->graph);
+    if (G->ws) dev_free(G->ctx, G->ws, G->ws_bytes);
     delete G;
     return HH_OK;
 }
@@ -1194,7 +1406,10 @@ extern "C" int hh_plan_set_operators(hh_plan *p, int nop, const double *coef) {
     // the host staging buffers feed the previous call's copies until they
     // complete; wait for the stream before refilling them
     HIPCHK(hipStreamSynchronize(c->stream));
+    const int old_real = p->real_op, old_nfx = p->ms.nfx;
     fill_table(p, nop, coef);
+    if (p->real_op != old_real || p->ms.nfx != old_nfx || nop != p->nop) p->gen++;
+    if ((rc = upload_modes(p))) return rc;
     const size_t cnt = (size_t)nc * n * nop;
     p->h32.resize(2 * cnt);
     for (size_t i = 0; i < cnt; i++) {

@@ -2,6 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stddef.h>
+#include <stdint.h>
+
+#include <complex>
+#include <vector>
 
 namespace hh {
 
@@ -58,8 +62,41 @@ struct TriHelm {
     int tiled = 0;      // field in the mode-tiled layout of the row DSTs: tile width (a multiple of W), 0 = row-major
     int real = 0;       // every coefficient real: use coefr, the real parts in pairs ([q][3] / [q][6] complex slots)
     const void *coefr = nullptr;
+    // Near-resonant modes (ModeSet below): the partitioned kernel does not
+    // store them (bit m % 32 of fbits[o * fwords + m / 32] set); they are
+    // solved by the sequential kernel from the factors in fac.
+    const uint32_t *fbits = nullptr;
+    int fwords = 0;
+    const int *foff = nullptr, *fmode = nullptr;   // per operator: entries foff[o] .. foff[o+1]-1, their modes
+    const void *fac = nullptr;                     // [entry][row](L, R, Up): double (real) or double2
+    int nfx = 0;                                   // most entries of one operator
 };
 hipError_t launch_tri_helm(const TriHelm &t, DstTypes ty, const void *in, void *out, hipStream_t st);
+// Sequential reference-order substitution of the near-resonant modes of t
+// (FP64 arithmetic; storage as ty.in / ty.out); run before launch_tri_helm
+// on the same buffer.  No-op when t.nfx == 0.
+hipError_t launch_tri_seq(const TriHelm &t, DstTypes ty, const void *in, void *out, hipStream_t st);
+
+// A mode whose condition estimate exceeds this is singular to working
+// precision (no correct digit is left in FP64).
+constexpr double HH_SINGULAR_KAPPA = 1e14;
+// Default Varah-bound threshold above which a mode is near-resonant.
+constexpr double HH_KAPPA_FLAG = 4096.0;
+
+// Classification of the modes of a plan's operators (hh_modes.hip).
+struct ModeSet {
+    int n = 0, nop = 0, words = 0, nfx = 0;
+    long nmodes = 0;
+    std::vector<int> foff, fmode;         // per operator offsets (nop + 1), flagged modes
+    std::vector<uint32_t> bits;           // [op][words]
+    std::vector<std::complex<double>> facc;   // [entry][row][3]
+    std::vector<double> facr;             // real parts (real operators only)
+    std::vector<int> singular;            // per operator
+    std::vector<double> kappa_worst;      // per operator: largest condition estimate of its flagged modes
+};
+// hcoef: the plan's device-order band table [op][q][6 | 12]
+void classify_modes(int dim, int Nx, int Ny, int Nz, int nop, const std::complex<double> *hcoef,
+                    const double *lamx, const double *lamy, bool real, double kappa_flag, ModeSet &ms);
 
 // Compact fourth-order derivative line solves (band 1, diag, 1) over an
 // extended-grid field; see CompactProb in hh_tri.hip for the index maps.

@@ -325,10 +325,27 @@ extern "C" int hh_server_submit(hh_server *S, const hh_request *r, long *ticket)
     q->r = *r;
     q->shape = Shape{r->dim, r->Nx, r->Ny, r->dim == 3 ? r->Nz : 1, r->precision};
     q->t_arrive = Clock::now();
+    // a request whose operator is singular to working precision (k^2 at an
+    // eigenvalue) fails alone, before it reaches a batch
+    std::vector<double> coef;
+    int sing = 0;
+    int rc = request_coef(*r, coef);
+    if (rc == HH_OK)
+        rc = hh_operator_modes(r->dim, r->Nx, r->Ny, r->dim == 3 ? r->Nz : 1, coef.data(), 4096.0, nullptr, nullptr,
+                               0, &sing, nullptr);
+    if (rc == HH_OK && sing) rc = HH_ERR_SINGULAR;
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->stop) return HH_ERR_ARG;
     q->id = S->next_id++;
     *ticket = q->id;
+    if (rc != HH_OK) {
+        q->status = rc;
+        q->done = true;
+        q->t_start = q->t_done = Clock::now();
+        S->reqs[q->id] = std::move(q);
+        S->cv_done.notify_all();
+        return HH_OK;
+    }
     S->queue.push_back(q.get());
     S->reqs[q->id] = std::move(q);
     S->cv_work.notify_all();

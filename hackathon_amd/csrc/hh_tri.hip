@@ -96,7 +96,11 @@ template <typename T, int DIM, bool REAL> struct HelmBands {
         }
         return md;
     }
+    // Bands in the reference's operation order with no fused multiply-adds
+    // (2D_Program/main_functions/LU.c:16-23, 3D_Program/main_functions/
+    // LU.c:34-47): the FP64 bands equal the reference's to the last bit.
     HH_DEV Band<OT> band(const Mode &md, int q) const {
+#pragma clang fp contract(off)
         Band<OT> r;
         const T ms = q > 0 ? (T)1 : (T)0, ps = q < n - 1 ? (T)1 : (T)0;
         if constexpr (REAL) {
@@ -148,6 +152,8 @@ template <typename T, typename Ti, typename To, int DIM, bool REAL> struct HelmP
     C2<To> *out;
     long field, S, nmodes;
     int tw;
+    const uint32_t *fbits;   // near-resonant modes, solved by k_tri_seq: not stored here
+    int fwords;
     struct Mode { typename HelmBands<T, DIM, REAL>::Mode bm; long off, step; };
     HH_DEV int n() const { return bands.n; }
     HH_DEV Mode mode(long m, int bt) const {
@@ -169,6 +175,14 @@ template <typename T, typename Ti, typename To, int DIM, bool REAL> struct HelmP
     }
     HH_DEV void store(int b, const Mode &md, int q, C2<T> x) const {
         out[b * field + md.off + (long)q * md.step] = cconv<To, T>(x);
+    }
+    HH_DEV bool skip(int o, long m) const { return fbits && ((fbits[(long)o * fwords + (m >> 5)] >> (m & 31)) & 1u); }
+    // FP64 access of the sequential kernel
+    HH_DEV double2 rhs64(int b, const Mode &md, int q) const {
+        return cconv<double, Ti>(in[b * field + md.off + (long)q * md.step]);
+    }
+    HH_DEV void store64(int b, const Mode &md, int q, double2 x) const {
+        out[b * field + md.off + (long)q * md.step] = cconv<To, double>(x);
     }
 };
 
@@ -220,6 +234,7 @@ template <typename T> struct CompactProb {
     HH_DEV void store(int bt, const Mode &md, int q, C2<T> x) const {
         out[bt * field_out + md.out + (long)q * oS] = x;
     }
+    HH_DEV bool skip(int, long) const { return false; }
 };
 
 // Blocks are dealt round-robin to the 8 XCDs (block b runs on XCD b % 8).
@@ -431,7 +446,7 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
     }
 #pragma unroll
     for (int r = 0; r < NR; r++) { D0[r] = PD[0][r]; D1[r] = PD[1][r]; }
-    if (!valid) return;
+    if (!valid || prob.skip(o, m)) return;
     // 4. back-substitution of the interior and store
 #pragma unroll
     for (int r = 0; r < NR; r++) {
@@ -448,6 +463,190 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
         }
         prob.store(b, md, e, xe);
     }
+}
+
+// Near-resonant modes: the reference's substitution (2D_Program/
+// solver_functions/solver.c:71-86, 3D_Program/solver_functions/
+// solver.c:356-369) with the factors the host computed in the order of its
+// LU() (L_q, R_q, Up_q; the back-substitution multiplies by R_q):
+//   y_0 = d_0,  y_q = d_q - L_q y_{q-1};   x_{n-1} = R y_{n-1},
+//   x_q = (y_q - Up_q x_{q+1}) R_q,
+// evaluated in double-double arithmetic (about 106 bits), so the result is
+// the exact substitution with the reference's pivots rounded once to FP64:
+// it differs from the reference's sequential FP64 substitution only by that
+// substitution's own rounding.  Both first-order recurrences are linear, so
+// they run in parallel: thread t of the block holds R consecutive rows,
+// runs its rows' recurrence from a zero start while carrying the product of
+// its multipliers (y_q = y0_q + g_q y_{s-1}), the chunks' affine maps
+// z -> y0_e + g_e z are combined by an inclusive scan over the block (wave
+// shuffles, then the wave totals through LDS), and each row is fixed up
+// with the value that enters its chunk; the backward pass scans from the
+// other end.  Block (x, y): entry foff[o] + x of operator o = b % nop,
+// right-hand side b = b0 + y, 64 NW <= 512 threads.
+struct DD {
+    double hi, lo;
+};
+HH_DEV DD dd(double a) { return DD{a, 0.0}; }
+HH_DEV DD two_sum(double a, double b) {
+    const double s = a + b, bb = s - a;
+    return DD{s, (a - (s - bb)) + (b - bb)};
+}
+HH_DEV DD quick(double s, double e) {
+    const double h = s + e;
+    return DD{h, e - (h - s)};
+}
+HH_DEV DD add(DD a, DD b) {
+    const DD s = two_sum(a.hi, b.hi);
+    return quick(s.hi, s.lo + (a.lo + b.lo));
+}
+HH_DEV DD neg(DD a) { return DD{-a.hi, -a.lo}; }
+HH_DEV DD mul(DD a, double b) {
+    const double p = a.hi * b;
+    const double e = __builtin_fma(a.hi, b, -p);
+    return quick(p, e + a.lo * b);
+}
+HH_DEV DD mul(DD a, DD b) {
+    const double p = a.hi * b.hi;
+    const double e = __builtin_fma(a.hi, b.hi, -p);
+    return quick(p, e + (a.hi * b.lo + a.lo * b.hi));
+}
+struct CDD {
+    DD re, im;
+};
+HH_DEV CDD cdd(double2 v) { return CDD{dd(v.x), dd(v.y)}; }
+HH_DEV CDD add(CDD a, CDD b) { return CDD{add(a.re, b.re), add(a.im, b.im)}; }
+HH_DEV CDD sub(CDD a, CDD b) { return CDD{add(a.re, neg(b.re)), add(a.im, neg(b.im))}; }
+HH_DEV CDD mul(CDD a, double s) { return CDD{mul(a.re, s), mul(a.im, s)}; }
+HH_DEV CDD mul(CDD a, double2 s) {
+    return CDD{add(mul(a.re, s.x), neg(mul(a.im, s.y))), add(mul(a.re, s.y), mul(a.im, s.x))};
+}
+HH_DEV CDD mul(CDD a, DD s) { return CDD{mul(a.re, s), mul(a.im, s)}; }
+HH_DEV CDD mul(CDD a, CDD s) {
+    return CDD{add(mul(a.re, s.re), neg(mul(a.im, s.im))), add(mul(a.re, s.im), mul(a.im, s.re))};
+}
+// multipliers: real (real operator) or complex
+template <typename OT> struct Mult;
+template <> struct Mult<double> {
+    using M = DD;
+    static HH_DEV M one() { return dd(1.0); }
+    static HH_DEV M zero() { return dd(0.0); }
+    static HH_DEV M scale(M g, double f) { return mul(g, -f); }   // -f g
+};
+template <> struct Mult<double2> {
+    using M = CDD;
+    static HH_DEV M one() { return CDD{dd(1.0), dd(0.0)}; }
+    static HH_DEV M zero() { return CDD{dd(0.0), dd(0.0)}; }
+    static HH_DEV M scale(M g, double2 f) { return mul(g, mk<double>(-f.x, -f.y)); }
+};
+HH_DEV double shfl_d(double v, int src) { return __shfl(v, src, 64); }
+HH_DEV DD shfl(DD v, int src) { return DD{shfl_d(v.hi, src), shfl_d(v.lo, src)}; }
+HH_DEV CDD shfl(CDD v, int src) { return CDD{shfl(v.re, src), shfl(v.im, src)}; }
+
+// affine map z -> B + A z
+template <typename M> struct Aff {
+    CDD B;
+    M A;
+};
+template <typename M> HH_DEV Aff<M> compose(const Aff<M> &f, const Aff<M> &g) {   // f o g
+    return Aff<M>{add(f.B, mul(g.B, f.A)), mul(f.A, g.A)};
+}
+template <typename M> HH_DEV Aff<M> shfl(const Aff<M> &f, int src) { return Aff<M>{shfl(f.B, src), shfl(f.A, src)}; }
+
+// Value entering each thread's chunk: forward (REV = false) the composition
+// of the maps of threads 0..t-1 applied to 0, backward that of threads
+// t+1..NT-1.  All threads of the block call it (two barriers).
+template <bool REV, typename M>
+HH_DEV CDD scan_in(Aff<M> f, Aff<M> *lds, int lane, int wave, int NW) {
+    // inclusive scan inside the wave
+    for (int k = 1; k < 64; k <<= 1) {
+        const int src = REV ? lane + k : lane - k;
+        const Aff<M> g = shfl(f, src < 0 || src > 63 ? lane : src);
+        if (REV ? lane + k < 64 : lane >= k) f = compose(f, g);
+    }
+    const int last = REV ? 0 : 63;
+    if (lane == last) lds[wave] = f;
+    __syncthreads();
+    // composition of the earlier waves' totals (in processing order)
+    Aff<M> pre = f;   // set below when an earlier wave exists
+    bool have = false;
+    if (!REV) {
+        for (int w = 0; w < wave; w++) { pre = have ? compose(lds[w], pre) : lds[w]; have = true; }
+    } else {
+        for (int w = NW - 1; w > wave; w--) { pre = have ? compose(lds[w], pre) : lds[w]; have = true; }
+    }
+    __syncthreads();
+    // exclusive value: the previous thread's inclusive map applied to the
+    // earlier waves' total, applied to 0
+    const int src = REV ? lane + 1 : lane - 1;
+    const Aff<M> prev = shfl(f, src < 0 || src > 63 ? lane : src);
+    const bool edge = REV ? lane == 63 : lane == 0;
+    const CDD zero{dd(0.0), dd(0.0)};
+    const CDD base = have ? pre.B : zero;
+    if (edge) return base;
+    return add(prev.B, mul(base, prev.A));
+}
+
+template <int R, class Prob, typename OT>
+__global__ __launch_bounds__(512) void k_tri_seq(Prob prob, const int *foff, const int *fmode, const OT *fac,
+                                                 int nop, int b0) {
+    using MT = Mult<OT>;
+    using M = typename MT::M;
+    __shared__ Aff<M> lds[8];
+    const int n = prob.n();
+    const int b = b0 + (int)blockIdx.y;
+    const int o = b % nop;
+    const int cnt = foff[o + 1] - foff[o];
+    if ((int)blockIdx.x >= cnt) return;   // uniform over the block
+    const int e = foff[o] + (int)blockIdx.x;
+    const auto md = prob.mode(fmode[e], b);
+    const OT *F = fac + (long)e * n * 3;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, NW = blockDim.x >> 6;
+    const int s = t * R;
+    CDD y[R];
+    M g[R];
+    OT Lf[R], Rf[R], Uf[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        const int q = min(s + i, n - 1);
+        y[i] = cdd(prob.rhs64(b, md, q));
+        Lf[i] = F[3 * q];
+        Rf[i] = F[3 * q + 1];
+        Uf[i] = F[3 * q + 2];
+    }
+    // forward, local: y0 from a zero start, g the product of -L
+    CDD yc{dd(0.0), dd(0.0)};
+    M gc = MT::one();
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        if (s + i < n) {
+            yc = sub(y[i], mul(yc, Lf[i]));   // L_0 = 0 starts the recurrence at row 0
+            gc = MT::scale(gc, Lf[i]);
+        }
+        y[i] = yc;
+        g[i] = gc;
+    }
+    CDD yin = scan_in<false, M>(Aff<M>{yc, gc}, lds, lane, wave, NW);
+#pragma unroll
+    for (int i = 0; i < R; i++) y[i] = add(y[i], mul(yin, g[i]));
+    // backward, local: x0 from a zero start at the chunk's end, h the product of -R Up
+    CDD xc{dd(0.0), dd(0.0)};
+    M hc = MT::one();
+#pragma unroll
+    for (int i = R - 1; i >= 0; i--) {
+        if (s + i < n) {
+            xc = mul(sub(y[i], mul(xc, Uf[i])), Rf[i]);   // Up_{n-1} = 0 starts it at the last row
+            hc = mul(MT::scale(hc, Uf[i]), Rf[i]);
+        }
+        y[i] = xc;
+        g[i] = hc;
+    }
+    CDD xin = scan_in<true, M>(Aff<M>{xc, hc}, lds, lane, wave, NW);
+#pragma unroll
+    for (int i = 0; i < R; i++)
+        if (s + i < n) {
+            const CDD x = add(y[i], mul(xin, g[i]));
+            prob.store64(b, md, s + i, mk<double>(x.re.hi + x.re.lo, x.im.hi + x.im.lo));
+        }
 }
 
 // LDS bytes of the reduced system: two buffers of A, C (operator values of
@@ -555,6 +754,8 @@ static hipError_t launch_tri_r(const TriHelm &t, const void *in, void *out, hipS
     pb.nmodes = nmodes;
     // tiles are powers of two holding whole blocks of W modes
     pb.tw = t.tiled;
+    pb.fbits = t.fbits;
+    pb.fwords = t.fwords;
     if (t.tiled && ((t.tiled & (t.tiled - 1)) || t.tiled % t.W)) return hipErrorInvalidValue;
     return launch_tri_prob<T, MM>(pb, pb.bands.n, nmodes, t.batch, pb.bands.nop, t.W, t.P, t.nr, st);
 }
@@ -573,6 +774,66 @@ static hipError_t launch_tri_mm(const TriHelm &t, const void *in, void *out, hip
     case 16: return launch_tri_t<T, Ti, To, 16>(t, in, out, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+template <int R, typename Ti, typename To, int DIM, bool REAL>
+static hipError_t launch_seq_r(const TriHelm &t, const void *in, void *out, hipStream_t st) {
+    using OT = typename std::conditional<REAL, double, double2>::type;
+    HelmProb<double, Ti, To, DIM, REAL> pb;
+    pb.bands.Nx = t.Nx;
+    pb.bands.coef = (const double2 *)(REAL ? t.coefr : t.coef);
+    pb.bands.nop = t.nop > 0 ? t.nop : 1;
+    pb.bands.cstride = REAL ? t.cstride / 2 : t.cstride;
+    pb.bands.lamx = t.lamx;
+    pb.bands.lamy = t.lamy;
+    pb.in = (const C2<Ti> *)in;
+    pb.out = (C2<To> *)out;
+    pb.field = t.field;
+    if (t.dim == 2) { pb.bands.n = t.Ny; pb.nmodes = t.Nx; pb.S = t.Nx; }
+    else { pb.bands.n = t.Nz; pb.nmodes = (long)t.Nx * t.Ny; pb.S = (long)t.Nx * t.Ny; }
+    pb.tw = t.tiled;
+    pb.fbits = nullptr;
+    pb.fwords = 0;
+    const int n = pb.bands.n;
+    const int NW = (n + 64 * R - 1) / (64 * R);
+    if (NW > 8) return hipErrorInvalidValue;
+    const int nop = pb.bands.nop;
+    for (int b0 = 0; b0 < t.batch; b0 += 65535) {
+        const int nb = t.batch - b0 < 65535 ? t.batch - b0 : 65535;
+        hipLaunchKernelGGL((k_tri_seq<R, HelmProb<double, Ti, To, DIM, REAL>, OT>), dim3((unsigned)t.nfx, (unsigned)nb),
+                           dim3(64 * NW), 0, st, pb, t.foff, t.fmode, (const OT *)t.fac, nop, b0);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+template <typename Ti, typename To, int DIM, bool REAL>
+static hipError_t launch_seq_d(const TriHelm &t, const void *in, void *out, hipStream_t st) {
+    // The double-double scan over the block costs about 1000 FP64 operations per
+    // thread against about 90 per row of the local recurrences, so blocks of
+    // about 128 threads (8 rows each at n = 1023) keep the work low while the
+    // critical path stays a few thousand cycles.
+    const int n = t.dim == 2 ? t.Ny : t.Nz;
+    if (n <= 128) return launch_seq_r<1, Ti, To, DIM, REAL>(t, in, out, st);
+    if (n <= 256) return launch_seq_r<2, Ti, To, DIM, REAL>(t, in, out, st);
+    if (n <= 512) return launch_seq_r<4, Ti, To, DIM, REAL>(t, in, out, st);
+    if (n <= 4096) return launch_seq_r<8, Ti, To, DIM, REAL>(t, in, out, st);
+    return launch_seq_r<16, Ti, To, DIM, REAL>(t, in, out, st);
+}
+
+template <typename Ti, typename To>
+static hipError_t launch_seq_t(const TriHelm &t, const void *in, void *out, hipStream_t st) {
+    if (t.dim == 2) return t.real ? launch_seq_d<Ti, To, 2, true>(t, in, out, st) : launch_seq_d<Ti, To, 2, false>(t, in, out, st);
+    return t.real ? launch_seq_d<Ti, To, 3, true>(t, in, out, st) : launch_seq_d<Ti, To, 3, false>(t, in, out, st);
+}
+
+hipError_t launch_tri_seq(const TriHelm &t, DstTypes ty, const void *in, void *out, hipStream_t st) {
+    if (t.nfx <= 0) return hipSuccess;
+    if (!t.foff || !t.fmode || !t.fac || t.batch < 1) return hipErrorInvalidValue;
+    if (ty.in == 0 && ty.out == 0) return launch_seq_t<double, double>(t, in, out, st);
+    if (ty.in == 1 && ty.out == 1) return launch_seq_t<float, float>(t, in, out, st);
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_compact(const Compact &cd, hipStream_t st) {

@@ -238,6 +238,40 @@ def robin3d(coef, Nx, Ny, Nz, h, lo=None, hi=None):
     return _robin(lib().hh_robin3d, (Nx, Ny, Nz), 12, coef, h, lo, hi)
 
 
+def _packed_operator(dim, Nx, Ny, Nz, coef):
+    nvec, n = (6, Ny) if dim == 2 else (12, Nz)
+    ops = _operators(coef, nvec, n)
+    if len(ops) != 1:
+        raise ValueError("one operator expected")
+    return np.ascontiguousarray(np.stack(ops[0]))
+
+
+def operator_modes(dim, Nx, Ny, Nz, coef, kappa=4096.0):
+    """Host-only classification of one operator's modes: (near-resonant modes, singular?, largest
+    condition estimate among them).  coef: 2D [bp, ap, b, a, bm, am], 3D the twelve vectors."""
+    packed = _packed_operator(dim, Nx, Ny, Nz, coef)
+    nf, sg, kw = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    _check(lib().hh_operator_modes(dim, Nx, Ny, Nz if dim == 3 else 1, _hptr(packed), ctypes.c_double(kappa),
+                                   ctypes.byref(nf), None, 0, ctypes.byref(sg), ctypes.byref(kw)))
+    buf = (ctypes.c_int * max(nf.value, 1))()
+    _check(lib().hh_operator_modes(dim, Nx, Ny, Nz if dim == 3 else 1, _hptr(packed), ctypes.c_double(kappa),
+                                   ctypes.byref(nf), buf, nf.value, ctypes.byref(sg), ctypes.byref(kw)))
+    return list(buf[:nf.value]), bool(sg.value), kw.value
+
+
+def operator_mode_factors(dim, Nx, Ny, Nz, coef, mode):
+    """Host-only reference-order factors of one mode: complex array (n, 3) of (L, R, Up), R the
+    back-substitution multiplier (2D 1/U, 3D the reference's reciprocal pivot).  Returns
+    (factors, singular?)."""
+    packed = _packed_operator(dim, Nx, Ny, Nz, coef)
+    n = Ny if dim == 2 else Nz
+    out = np.zeros((n, 3), np.complex128)
+    rc = lib().hh_operator_mode_factors(dim, Nx, Ny, Nz if dim == 3 else 1, _hptr(packed), int(mode), _hptr(out))
+    if rc not in (HH_OK, HH_ERR_SINGULAR):
+        _check(rc)
+    return out, rc == HH_ERR_SINGULAR
+
+
 # ------------------------------------------------------------------ plans
 def _operators(coef, nvec, n):
     """Normalise one operator (nvec vectors of length n) or a list of them."""
@@ -314,6 +348,26 @@ class Plan:
     @property
     def is_real_operator(self):
         return bool(lib().hh_plan_is_real_operator(self.h))
+
+    def set_resonance_threshold(self, kappa):
+        """Varah-bound threshold above which a mode is near-resonant (default 4096): such modes
+        run the reference's sequential substitution with host-computed reference-order factors."""
+        self.ctx.bind_stream()
+        _check(lib().hh_plan_set_resonance_threshold(self.h, ctypes.c_double(kappa)))
+
+    def mode_info(self, op=0):
+        """(number of near-resonant modes, singular?, largest condition estimate among them) of operator op."""
+        f, sg, k = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().hh_plan_mode_info(self.h, int(op), ctypes.byref(f), ctypes.byref(sg), ctypes.byref(k)))
+        return f.value, bool(sg.value), k.value
+
+    def flagged_modes(self, op=0):
+        n = lib().hh_plan_flagged_modes(self.h, int(op), None, 0)
+        if n < 0:
+            _check(n)
+        buf = (ctypes.c_int * max(n, 1))()
+        lib().hh_plan_flagged_modes(self.h, int(op), buf, n)
+        return list(buf[:n])
 
     def set_operators(self, coef):
         """Replace the operators (at most as many as the plan was created with)."""

@@ -29,7 +29,7 @@ enum {
     HH_ERR_DEVICE = 2,       /* HIP runtime failure */
     HH_ERR_NOMEM = 3,        /* device allocation failed or over budget */
     HH_ERR_UNSUPPORTED = 4,  /* shape outside the kernels' limits */
-    HH_ERR_SINGULAR = 5      /* a mode's pivot vanished (resonance) */
+    HH_ERR_SINGULAR = 5      /* an operator is singular to working precision (resonance) */
 };
 
 enum { HH_PREC_F64 = 0, HH_PREC_MIXED = 1 };
@@ -160,6 +160,41 @@ int hh_plan_last_refine(const hh_plan *plan, int *sweeps, double *rel_residual);
  * coef has the layout of the create_multi call.  Waits for the work already
  * queued on the context's stream, then queues the table copies on it. */
 int hh_plan_set_operators(hh_plan *plan, int nop, const double *coef);
+/* Near-resonant modes (no reference counterpart; the reference's LU() divides
+ * by whatever pivot it meets, 2D_Program/main_functions/LU.c:36,
+ * 3D_Program/main_functions/LU.c:68).  For every operator and mode the plan
+ * bounds the condition number of the mode's tridiagonal system by Varah's
+ * bound max_q(|sub|+|dia|+|sup|) / min_q(|dia|-|sub|-|sup|).  Modes that are
+ * not diagonally dominant or whose bound exceeds kappa (default 4096) are
+ * near-resonant: their LU factors are computed on the host in the
+ * reference's operation order and the device runs the reference's
+ * sequential substitution on them (FP64 arithmetic, also in mixed
+ * precision); the other modes use the partitioned elimination.  A
+ * near-resonant mode whose estimated condition number exceeds 1e14, or
+ * whose pivot is zero or not finite, makes its operator singular: hh_solve
+ * (and hh_graph_create) then returns HH_ERR_SINGULAR without touching the
+ * output when a right-hand side of the batch uses that operator.
+ * set_resonance_threshold re-classifies the current operators (kappa = 0
+ * routes every mode to the sequential substitution, a huge kappa none but
+ * the indefinite ones).  mode_info: number of near-resonant modes of
+ * operator op, whether it is singular, and the largest condition estimate
+ * among its near-resonant modes.  flagged_modes writes up to cap of their
+ * mode indices and returns their number. */
+int hh_plan_set_resonance_threshold(hh_plan *plan, double kappa);
+/* Host only (no device): the same classification of one operator, coef in
+ * the layout of hh_plan*_create_multi with nop = 1.  Writes up to cap of the
+ * near-resonant modes, their number, whether the operator is singular and
+ * the largest condition estimate among its near-resonant modes. */
+int hh_operator_modes(int dim, int Nx, int Ny, int Nz, const double *coef, double kappa, int *nflagged,
+                      int *modes, int cap, int *singular, double *kappa_worst);
+/* Host only: the reference-order factors of one mode, 3n complex values
+ * (L_q, R_q, Up_q) for q = 0..n-1, L = the reference's L, Up its Up (0 on
+ * the last row), R the back-substitution multiplier (2D: 1/U, 3D: the
+ * reference's stored reciprocal U).  Returns HH_ERR_SINGULAR (factors still
+ * written) when the operator is singular to working precision. */
+int hh_operator_mode_factors(int dim, int Nx, int Ny, int Nz, const double *coef, int mode, double *factors);
+int hh_plan_mode_info(const hh_plan *plan, int op, int *flagged, int *singular, double *kappa);
+int hh_plan_flagged_modes(const hh_plan *plan, int op, int *modes, int cap);
 /* Block shape of the tridiagonal kernel: W neighbouring modes per block,
  * rows_per_thread in {4, 8, 16} (the chunk count follows), rhs_per_thread
  * in {1, 2, 4} right-hand sides of one operator handled together by a
@@ -183,7 +218,8 @@ int hh_plan_set_tile_width(hh_plan *plan, int tw);
 /* Mode solves with real bands.  When every stencil coefficient of the
  * plan's operators has a zero imaginary part (real k), the tridiagonal
  * kernels run with real band arithmetic (complex right-hand sides); this
- * is chosen again by every hh_plan_set_operators.  mode -1 = automatic
+ * is chosen again by every hh_plan_set_operators (a recorded graph whose
+ * plan changes band kind refuses to replay).  mode -1 = automatic
  * (default), 0 = complex-band kernels always.  Both give the oracle's
  * solution to the same tolerance; they differ in rounding and speed.
  * No counterpart in the reference (its LU factors are complex,
@@ -197,9 +233,14 @@ int hh_plan_is_real_operator(const hh_plan *plan);
  * batches of uneven problems, e.g. one plan per grid size.  Solve i runs
  * plans[i] on batch[i] right-hand sides d_rhs[i] -> d_sol[i].  All plans
  * belong to ctx; mixed-precision plans must use fixed sweeps
- * (hh_plan_set_refine(plan, 0, n)).  Tables and scratch are built at
- * creation, so a launch allocates nothing; the buffers must stay valid and
- * the solves of one graph share the context's scratch in sequence. */
+ * (hh_plan_set_refine(plan, 0, n)).  Tables are built at creation and the
+ * graph owns its workspace (freed by hh_graph_destroy), so a launch
+ * allocates nothing and later solves on the context do not disturb it; the
+ * buffers must stay valid.  Operator values changed by
+ * hh_plan_set_operators reach later replays when nothing structural
+ * changed; a plan whose band kind, operator count, near-resonant mode count,
+ * device tables or shape settings changed since the recording makes
+ * hh_graph_launch return HH_ERR_ARG (record the graph again). */
 typedef struct hh_graph hh_graph;
 int hh_graph_create(hh_context *ctx, int nsolves, hh_plan *const *plans, void *const *d_rhs, void *const *d_sol,
                     const int *batch, hh_graph **graph);

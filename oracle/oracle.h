@@ -79,6 +79,13 @@ void or2d_LU(int Nx, int Ny, const ocplx *bp, const ocplx *ap, const ocplx *b,
 /* solver.c:14-117: sol = S_x T^{-1} S_x rhs, using the stored factors. */
 void or2d_solver(int Nx, int Ny, const ocplx *rhs, const ocplx *L,
                  const ocplx *U, const ocplx *Up, ocplx *sol, int threads);
+/* Steady state: the same solves with caller-held DST plans (plan creation
+ * outside the timed call).  or_plan(N) makes the DST-I plan of length N. */
+typedef struct or_dstplan or_dstplan;
+or_dstplan *or_dst_plan(int N);
+void or_dst_plan_free(or_dstplan *plan);
+void or2d_solver_planned(const or_dstplan *plan, int Nx, int Ny, const ocplx *rhs, const ocplx *L,
+                         const ocplx *U, const ocplx *Up, ocplx *sol, int threads);
 /* residual.c:3-34: res = A sol - rhs (9-point operator, zero ghosts). */
 void or2d_residual(int Nx, int Ny, const ocplx *bp, const ocplx *ap,
                    const ocplx *b, const ocplx *a, const ocplx *bm,
@@ -101,6 +108,8 @@ void or3d_LU(int Nx, int Ny, int Nz, const ocplx *coef, ocplx *L, ocplx *U,
 /* solver.c:263-413 (CPU branch) */
 void or3d_solver(int Nx, int Ny, int Nz, const ocplx *rhs, const ocplx *L,
                  const ocplx *U, const ocplx *Up, ocplx *sol, int threads);
+void or3d_solver_planned(const or_dstplan *px, const or_dstplan *py, int Nx, int Ny, int Nz, const ocplx *rhs,
+                         const ocplx *L, const ocplx *U, const ocplx *Up, ocplx *sol, int threads);
 /* residual.c:42-96 (27-point operator, zero ghosts) */
 void or3d_residual(int Nx, int Ny, int Nz, const ocplx *coef, const ocplx *sol,
                    const ocplx *rhs, ocplx *res);

@@ -33,6 +33,8 @@ def lib():
         _lib.or_normInf.restype = d
         _lib.or_normL2.restype = d
         _lib.or_wtime.restype = d
+        _lib.or_dst_plan.restype = ctypes.c_void_p
+        _lib.or_dst_plan_free.argtypes = [ctypes.c_void_p]
     return _lib
 
 
@@ -90,6 +92,27 @@ def solver2d(Nx, Ny, rhs, LUU, threads=1):
     sol = np.zeros(Nx * Ny, np.complex128)
     rhs = np.ascontiguousarray(rhs, dtype=np.complex128)
     lib().or2d_solver(Nx, Ny, _p(rhs), *[_p(a) for a in LUU], _p(sol), threads)
+    return sol
+
+
+class DstPlan:
+    """Caller-held DST-I plan of length N (steady-state timing: plan creation outside the solve)."""
+
+    def __init__(self, N):
+        self.N = N
+        self.h = ctypes.c_void_p(lib().or_dst_plan(N))
+
+    def __del__(self):
+        try:
+            lib().or_dst_plan_free(self.h)
+        except Exception:
+            pass
+
+
+def solver2d_planned(plan, Nx, Ny, rhs, LUU, threads=1):
+    sol = np.zeros(Nx * Ny, np.complex128)
+    rhs = np.ascontiguousarray(rhs, dtype=np.complex128)
+    lib().or2d_solver_planned(plan.h, Nx, Ny, _p(rhs), *[_p(a) for a in LUU], _p(sol), threads)
     return sol
 
 
@@ -156,6 +179,13 @@ def solver3d(Nx, Ny, Nz, rhs, LUU, threads=1):
     sol = np.zeros(Nx * Ny * Nz, np.complex128)
     rhs = np.ascontiguousarray(rhs, dtype=np.complex128)
     lib().or3d_solver(Nx, Ny, Nz, _p(rhs), *[_p(a) for a in LUU], _p(sol), threads)
+    return sol
+
+
+def solver3d_planned(px, py, Nx, Ny, Nz, rhs, LUU, threads=1):
+    sol = np.zeros(Nx * Ny * Nz, np.complex128)
+    rhs = np.ascontiguousarray(rhs, dtype=np.complex128)
+    lib().or3d_solver_planned(px.h, py.h, Nx, Ny, Nz, _p(rhs), *[_p(a) for a in LUU], _p(sol), threads)
     return sol
 
 

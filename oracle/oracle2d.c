@@ -231,14 +231,22 @@ void or2d_LU(int Nx, int Ny, const ocplx *bp, const ocplx *ap, const ocplx *b,
     }
 }
 
-/* solver.c:14-117 + DST.c:6-44 */
+/* solver.c:14-117 + DST.c:6-44, with the DST plan created inside the call
+ * as the reference does (:56-64) */
 void or2d_solver(int Nx, int Ny, const ocplx *rhs, const ocplx *L,
                  const ocplx *U, const ocplx *Up, ocplx *sol, int threads) {
+    or_dstplan *plan = or_dst_plan(Nx);                              /* :56-64 */
+    or2d_solver_planned(plan, Nx, Ny, rhs, L, U, Up, sol, threads);
+    or_dst_plan_free(plan);
+}
+
+/* the same solve with a caller-held plan (steady state) */
+void or2d_solver_planned(const or_dstplan *plan, int Nx, int Ny, const ocplx *rhs, const ocplx *L,
+                         const ocplx *U, const ocplx *Up, ocplx *sol, int threads) {
     if (threads < 1) threads = 1;
     long Nxy = (long)Nx * Ny;
     ocplx *rhat = (ocplx *)malloc(sizeof(ocplx) * Nxy);
     ocplx *xhat = (ocplx *)malloc(sizeof(ocplx) * Nxy);
-    or_dstplan *plan = or_dst_plan(Nx);                              /* :56-64 */
     or_dst_lines_plan(plan, Ny, rhs, Nx, 1, rhat, Nx, 1, threads);   /* forwardDST */
 #pragma omp parallel for num_threads(threads)
     for (int i = 0; i < Nx; i++) {                                   /* :71-86 */
@@ -252,7 +260,6 @@ void or2d_solver(int Nx, int Ny, const ocplx *rhs, const ocplx *L,
     }
     /* reverseDST: row j gathers xhat[j + i*Ny] (stride Ny) */
     or_dst_lines_plan(plan, Ny, xhat, 1, Ny, sol, Nx, 1, threads);
-    or_dst_plan_free(plan);
     free(rhat);
     free(xhat);
 }

@@ -301,14 +301,22 @@ static void dst2d_layer(const or_dstplan *px, const or_dstplan *py, int Nx, int 
     or_dst_lines_plan(py, Nx, tmp, 1, Nx, out, 1, Nx, 1);
 }
 
-/* solver.c:263-413 (CPU branch) */
+/* solver.c:263-413 (CPU branch), plans created inside the call (:289-331) */
 void or3d_solver(int Nx, int Ny, int Nz, const ocplx *rhs, const ocplx *L,
                  const ocplx *U, const ocplx *Up, ocplx *sol, int threads) {
+    or_dstplan *px = or_dst_plan(Nx), *py = or_dst_plan(Ny);        /* :289-331 */
+    or3d_solver_planned(px, py, Nx, Ny, Nz, rhs, L, U, Up, sol, threads);
+    or_dst_plan_free(px);
+    or_dst_plan_free(py);
+}
+
+/* the same solve with caller-held plans (steady state) */
+void or3d_solver_planned(const or_dstplan *px, const or_dstplan *py, int Nx, int Ny, int Nz, const ocplx *rhs,
+                         const ocplx *L, const ocplx *U, const ocplx *Up, ocplx *sol, int threads) {
     if (threads < 1) threads = 1;
     long Nxy = (long)Nx * Ny, Nxz = (long)Nx * Nz;
     ocplx *rhat = (ocplx *)malloc(sizeof(ocplx) * Nxy * Nz);
     ocplx *xhat = (ocplx *)malloc(sizeof(ocplx) * Nxy * Nz);
-    or_dstplan *px = or_dst_plan(Nx), *py = or_dst_plan(Ny);        /* :289-331 */
 #pragma omp parallel num_threads(threads)
     {
         ocplx *tmp = (ocplx *)malloc(sizeof(ocplx) * Nxy);
@@ -330,8 +338,6 @@ void or3d_solver(int Nx, int Ny, int Nz, const ocplx *rhs, const ocplx *L,
         for (int l = 0; l < Nz; l++) dst2d_layer(px, py, Nx, Ny, xhat + l, Nz, tmp, sol + l * Nxy);       /* :376-388 */
         free(tmp);
     }
-    or_dst_plan_free(px);
-    or_dst_plan_free(py);
     free(rhat);
     free(xhat);
 }

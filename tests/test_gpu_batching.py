@@ -211,3 +211,60 @@ def test_spectrum_layouts_same_bits(H, ctx, oracle):
     with pytest.raises(H.HHError):   # a tile narrower than the block is refused at launch
         H._check(H.lib().hh_plan_set_tile_width(plan.h, 2))
         plan.solve(rhs)
+
+
+def test_graph_keeps_own_workspace(H, ctx, oracle):
+    """A graph owns its workspace: an eager solve that grows (and moves) the
+    context's scratch afterwards must not disturb later replays."""
+    import torch
+    N = 127
+    s = oracle.solve2d(6, N, 1, [30.0, 0.0, 3, 2, 0.5], threads=8)
+    plan = H.Plan2D(ctx, N, N, s["coef"], precision=H.PREC_MIXED)
+    plan.set_refine(0.0, 2)
+    rhs = dev(np.stack([s["rhs"], 0.25 * s["rhs"]]))
+    out = torch.empty_like(rhs)
+    eager = plan.solve(rhs).cpu().numpy()
+    g = H.Graph(ctx, [(plan, rhs, out)])
+    big = H.Plan2D(ctx, 255, 255, oracle.solve2d(6, 255, 1, [30.0, 0.0, 3, 2, 0.5], threads=8)["coef"],
+                   precision=H.PREC_MIXED)
+    big_rhs = dev(np.ones(16 * 255 * 255))
+    for rep in range(3):
+        out.zero_()
+        g.launch()
+        torch.cuda.synchronize()
+        assert np.array_equal(out.cpu().numpy(), eager), f"replay {rep}"
+        # eager solves that need more workspace than the graph's solve
+        big.solve(big_rhs)
+        torch.cuda.synchronize()
+    say(f"graph replays unchanged after the context scratch grew to {ctx.bytes_in_use()} bytes in use")
+    g.close()
+
+
+def test_graph_refuses_changed_plan(H, ctx, oracle):
+    """A plan whose band kind changed since the recording refuses to replay;
+    new operator values of the same kind reach the replay."""
+    import torch
+    N = 63
+    s1 = oracle.solve2d(4, N, 1, [2.0, 0.0, 1, 1, 0.0], threads=8)
+    s2 = oracle.solve2d(4, N, 1, [2.5, 0.0, 2, 1, 0.0], threads=8)
+    plan = H.Plan2D(ctx, N, N, [s1["coef"]])
+    # k < pi: every mode diagonally dominant; with no condition threshold
+    # neither operator has a near-resonant mode, so only values change
+    plan.set_resonance_threshold(1e300)
+    rhs = dev(s2["rhs"])
+    out = torch.empty_like(rhs)
+    g = H.Graph(ctx, [(plan, rhs, out)])
+    assert plan.mode_info()[0] == 0
+    plan.set_operators([s2["coef"]])          # same band kind, no near-resonant mode either
+    assert plan.mode_info()[0] == 0
+    g.launch()
+    torch.cuda.synchronize()
+    err = relerr(out.cpu().numpy(), s2["sol"])
+    say(f"replay after set_operators (same kind): rel err {err:.2e}")
+    assert err <= 1e-12
+    cplx = [c * (1 + 0.01j) for c in s2["coef"]]
+    plan.set_operators([cplx])                # complex bands now: the graph is stale
+    with pytest.raises(H.HHError) as ei:
+        g.launch()
+    assert ei.value.status == H.HH_ERR_ARG
+    g.close()
