@@ -24,6 +24,17 @@ using cd = std::complex<double>;
 
 static thread_local std::string g_err;
 
+// Workspace of hh_solve_noalloc, for callers that cannot allocate (code under
+// HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+// one array per device, held until the process ends.  In host code the
+// array's name is a host-side stand-in: its device address is looked up once
+// per device at context creation (hipGetSymbolAddress) and only that address
+// is ever passed to kernels.
+#define HH_NOALLOC_BYTES ((size_t)128 << 20)
+static __device__ __align__(256) unsigned char noalloc_scratch[HH_NOALLOC_BYTES];
+static std::mutex g_noalloc_mu;
+static std::map<int, std::pair<void *, hipError_t>> g_noalloc;   // device -> (address, lookup status)
+
 static int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 static int fail(int code, const char *fmt, ...) {
     char buf[512];
@@ -54,6 +65,9 @@ struct hh_context {
     size_t scratch_bytes = 0;
     double *d_norm = nullptr;
     size_t budget = 0, in_use = 0;
+    void *noalloc_ws = nullptr;         // this device's noalloc_scratch (device address)
+    hipError_t noalloc_status = hipSuccess;
+    bool no_alloc = false;              // inside hh_solve_noalloc: allocations are refused
     std::mutex mu;
     // kernel timing (hh_context_enable_timing)
     struct Rec { hipEvent_t a, b; int cat; double bytes; };
@@ -69,7 +83,7 @@ struct TimeScope {
     hh_context *c;
     hh_context::Rec r;
     bool on;
-    TimeScope(hh_context *ctx, int cat, double bytes) : c(ctx), on(ctx->timing) {
+    TimeScope(hh_context *ctx, int cat, double bytes) : c(ctx), on(ctx->timing && !ctx->no_alloc) {
         if (!on) return;
         r.cat = cat;
         r.bytes = bytes;
@@ -134,6 +148,9 @@ struct hh_plan {
 
 // ------------------------------------------------------------------ helpers
 static int dev_alloc(hh_context *c, void **p, size_t bytes) {
+    if (c->no_alloc)
+        return fail(HH_ERR_UNSUPPORTED, "hh_solve_noalloc: a table of %zu bytes is missing (create the plan with the "
+                                        "context's current DST algorithm, or solve once with hh_solve first)", bytes);
     if (c->budget && c->in_use + bytes > c->budget)
         return fail(HH_ERR_NOMEM, "allocation of %zu bytes exceeds the memory budget (%zu of %zu in use)", bytes,
                     c->in_use, c->budget);
@@ -347,6 +364,19 @@ extern "C" int hh_context_create(int device, hh_context **out) {
     e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
     if (e != hipSuccess) { delete c; return fail(HH_ERR_DEVICE, "stream creation failed: %s", hipGetErrorString(e)); }
     c->stream = c->own;
+    {
+        // the static workspace of hh_solve_noalloc: its address on this device,
+        // looked up once per device with the device current
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        auto it = g_noalloc.find(device);
+        if (it == g_noalloc.end()) {
+            void *ptr = nullptr;
+            const hipError_t se = hipGetSymbolAddress(&ptr, HIP_SYMBOL(noalloc_scratch));
+            it = g_noalloc.emplace(device, std::make_pair(se == hipSuccess ? ptr : nullptr, se)).first;
+        }
+        c->noalloc_ws = it->second.first;
+        c->noalloc_status = it->second.second;
+    }
     if (hipMalloc((void **)&c->d_norm, hh::norm_buffer_doubles() * sizeof(double)) != hipSuccess) {
         (void)hipStreamDestroy(c->own);
         delete c;
@@ -810,6 +840,8 @@ extern "C" int hh_operator_mode_factors(int dim, int Nx, int Ny, int Nz, const d
     return ms.singular[0] ? fail(HH_ERR_SINGULAR, "operator is singular to working precision") : HH_OK;
 }
 
+static int prepare_tables(hh_plan *p);
+
 static int plan_check_dims(int dim, int Nx, int Ny, int Nz) {
     if (Nx < 2 || Ny < 2 || (dim == 3 && Nz < 2))
         return fail(HH_ERR_ARG, "grid sizes must be at least 2 (got %d x %d x %d)", Nx, Ny, Nz);
@@ -845,6 +877,8 @@ extern "C" int hh_plan2d_create_multi(hh_context *ctx, int Nx, int Ny, int nop, 
     eigenvalues(Nx, p->hlamx);
     fill_table(p, nop, coef);
     if ((rc = upload_plan(p, 6, Ny))) { hh_plan_destroy(p); return rc; }
+    // the transform tables the plan's solves read: a solve then allocates nothing but workspace
+    if ((rc = prepare_tables(p))) { hh_plan_destroy(p); return rc; }
     *out = p;
     return HH_OK;
 }
@@ -875,6 +909,8 @@ extern "C" int hh_plan3d_create_multi(hh_context *ctx, int Nx, int Ny, int Nz, i
     eigenvalues(Ny, p->hlamy);
     fill_table(p, nop, coef);
     if ((rc = upload_plan(p, 12, Nz))) { hh_plan_destroy(p); return rc; }
+    // the transform tables the plan's solves read: a solve then allocates nothing but workspace
+    if ((rc = prepare_tables(p))) { hh_plan_destroy(p); return rc; }
     *out = p;
     return HH_OK;
 }
@@ -1256,6 +1292,28 @@ static int solve_locked(hh_plan *p, const void *rhs, void *sol, int batch) {
     size_t ws_bytes = hh_plan_workspace_bytes(p, batch);
     if (ws_bytes && (rc = ensure_scratch(c, ws_bytes))) return rc;
     return solve_ws(p, rhs, sol, batch, c->scratch);
+}
+
+extern "C" int hh_solve_noalloc(hh_plan *p, const void *rhs, void *sol, int batch) {
+    if (!p || !rhs || !sol) return fail(HH_ERR_ARG, "hh_solve_noalloc: null argument");
+    if (batch < 1) return fail(HH_ERR_ARG, "hh_solve_noalloc: batch must be >= 1 (got %d)", batch);
+    hh_context *c = p->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = check_singular(p, batch);
+    if (rc) return rc;
+    if ((rc = set_device(c))) return rc;
+    if (!c->noalloc_ws)
+        return fail(HH_ERR_DEVICE, "hh_solve_noalloc: no address for the static workspace on device %d: %s", c->device,
+                    hipGetErrorString(c->noalloc_status));
+    const size_t ws = hh_plan_workspace_bytes(p, batch);
+    if (ws > HH_NOALLOC_BYTES)
+        return fail(HH_ERR_UNSUPPORTED, "hh_solve_noalloc: the solve needs %zu bytes of workspace, above the %zu of "
+                                        "its static workspace (use hh_solve, or a smaller batch)",
+                    ws, HH_NOALLOC_BYTES);
+    c->no_alloc = true;
+    rc = solve_ws(p, rhs, sol, batch, c->noalloc_ws);
+    c->no_alloc = false;
+    return rc;
 }
 
 extern "C" int hh_solve(hh_plan *p, const void *rhs, void *sol, int batch) {

@@ -147,6 +147,17 @@ size_t hh_plan_workspace_bytes(const hh_plan *plan, int batch);
  * residual until ||r||/||rhs|| <= tol or max_iter sweeps (hh_plan_set_refine). */
 int hh_solve(hh_plan *plan, const void *d_rhs, void *d_sol, int batch);
 int hh_solve_host(hh_plan *plan, const double *h_rhs, double *h_sol, int batch);
+/* hh_solve that allocates nothing.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same kernels and
+ * results as hh_solve; the workspace (hh_plan_workspace_bytes) comes from a
+ * static array of 128 MiB per device, held until the process ends, instead
+ * of the context's scratch.  A solve needing more returns
+ * HH_ERR_UNSUPPORTED; a plan table missing (the context's DST algorithm
+ * changed after the plan was created) returns HH_ERR_UNSUPPORTED too.  Every
+ * call on one device shares that array: calls that overlap on one GPU (two
+ * contexts or streams of one device running at the same time) race. */
+int hh_solve_noalloc(hh_plan *plan, const void *d_rhs, void *d_sol, int batch);
 /* tol > 0: refine until ||r||inf/||rhs||inf <= tol or max_iter sweeps (each
  * check reads two norms back to the host).  tol = 0: exactly max_iter sweeps
  * with no norm and no host synchronisation, the form a HIP graph can hold. */
