@@ -139,6 +139,8 @@ struct hh_plan {
     int *d_foff = nullptr, *d_fmode = nullptr;
     uint32_t *d_bits = nullptr;
     void *d_facc = nullptr, *d_facr = nullptr;
+    std::vector<cd> hfacc_t;                     // staging of the device layout
+    std::vector<double> hfacr_t;
     size_t cap_foff = 0, cap_fmode = 0, cap_bits = 0, cap_facc = 0, cap_facr = 0;
     // Generation: bumped whenever a change would invalidate a recorded graph
     // (device tables reallocated, band kind, operator count or launch shapes
@@ -743,8 +745,19 @@ static int ensure_dev(hh_plan *p, void **ptr, size_t *cap, size_t bytes) {
 // fill_table, which waits for the stream first).
 static int upload_modes(hh_plan *p) {
     hh_context *c = p->ctx;
-    const hh::ModeSet &m = p->ms;
+    hh::ModeSet &m = p->ms;
     int rc;
+    // device layout of the factors: [row][entry](L, P, R, Up), so that the
+    // lanes of neighbouring entries read neighbouring records
+    const long ne = (long)m.fmode.size(), n = m.n;
+    p->hfacc_t.resize(m.facc.size());
+    p->hfacr_t.resize(m.facr.size());
+    for (long e = 0; e < ne; e++)
+        for (long q = 0; q < n; q++)
+            for (int k = 0; k < 4; k++) {
+                p->hfacc_t[(q * ne + e) * 4 + k] = m.facc[(e * n + q) * 4 + k];
+                if (!m.facr.empty()) p->hfacr_t[(q * ne + e) * 4 + k] = m.facr[(e * n + q) * 4 + k];
+            }
     if ((rc = ensure_dev(p, (void **)&p->d_foff, &p->cap_foff, m.foff.size() * sizeof(int)))) return rc;
     if ((rc = ensure_dev(p, (void **)&p->d_fmode, &p->cap_fmode, m.fmode.size() * sizeof(int)))) return rc;
     if ((rc = ensure_dev(p, (void **)&p->d_bits, &p->cap_bits, m.bits.size() * sizeof(uint32_t)))) return rc;
@@ -757,8 +770,8 @@ static int upload_modes(hh_plan *p) {
     if ((rc = cp(p->d_foff, m.foff.data(), m.foff.size() * sizeof(int)))) return rc;
     if ((rc = cp(p->d_fmode, m.fmode.data(), m.fmode.size() * sizeof(int)))) return rc;
     if ((rc = cp(p->d_bits, m.bits.data(), m.bits.size() * sizeof(uint32_t)))) return rc;
-    if ((rc = cp(p->d_facc, m.facc.data(), m.facc.size() * sizeof(cd)))) return rc;
-    return cp(p->d_facr, m.facr.data(), m.facr.size() * sizeof(double));
+    if ((rc = cp(p->d_facc, p->hfacc_t.data(), p->hfacc_t.size() * sizeof(cd)))) return rc;
+    return cp(p->d_facr, p->hfacr_t.data(), p->hfacr_t.size() * sizeof(double));
 }
 
 static int upload_plan(hh_plan *p, int ncoef, int n) {
@@ -835,8 +848,14 @@ extern "C" int hh_operator_mode_factors(int dim, int Nx, int Ny, int Nz, const d
     hh::ModeSet ms;
     const bool real = table_real(tab);
     hh::classify_modes(dim, Nx, Ny, Nz, 1, tab.data(), lx.data(), ly.data(), real, 0.0, ms);
-    const cd *f = ms.facc.data() + (size_t)mode * n * 3;
-    for (long i = 0; i < 3L * n; i++) { factors[2 * i] = f[i].real(); factors[2 * i + 1] = f[i].imag(); }
+    const cd *f = ms.facc.data() + (size_t)mode * n * 4;
+    for (long q = 0; q < n; q++) {   // (L, R, Up) of each row
+        const cd v[3] = {f[4 * q], f[4 * q + 2], f[4 * q + 3]};
+        for (int k = 0; k < 3; k++) {
+            factors[2 * (3 * q + k)] = v[k].real();
+            factors[2 * (3 * q + k) + 1] = v[k].imag();
+        }
+    }
     return ms.singular[0] ? fail(HH_ERR_SINGULAR, "operator is singular to working precision") : HH_OK;
 }
 
@@ -1133,6 +1152,7 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
         t.foff = p->d_foff;
         t.fmode = p->d_fmode;
         t.fac = t.real ? p->d_facr : p->d_facc;
+        t.nent = (long)p->ms.fmode.size();
     }
     // The two kernels touch disjoint outputs (the partitioned kernel reads the
     // near-resonant modes' rows but does not store them).  Run concurrently on

@@ -68,8 +68,9 @@ struct TriHelm {
     const uint32_t *fbits = nullptr;
     int fwords = 0;
     const int *foff = nullptr, *fmode = nullptr;   // per operator: entries foff[o] .. foff[o+1]-1, their modes
-    const void *fac = nullptr;                     // [entry][row](L, R, Up): double (real) or double2
+    const void *fac = nullptr;                     // [row][entry](L, P, R, Up): double (real) or double2
     int nfx = 0;                                   // most entries of one operator
+    long nent = 0;                                 // entries of all operators
 };
 hipError_t launch_tri_helm(const TriHelm &t, DstTypes ty, const void *in, void *out, hipStream_t st);
 // Sequential reference-order substitution of the near-resonant modes of t
@@ -89,7 +90,7 @@ struct ModeSet {
     long nmodes = 0;
     std::vector<int> foff, fmode;         // per operator offsets (nop + 1), flagged modes
     std::vector<uint32_t> bits;           // [op][words]
-    std::vector<std::complex<double>> facc;   // [entry][row][3]
+    std::vector<std::complex<double>> facc;   // [entry][row](L, P, R, Up)
     std::vector<double> facr;             // real parts (real operators only)
     std::vector<int> singular;            // per operator
     std::vector<double> kappa_worst;      // per operator: largest condition estimate of its flagged modes

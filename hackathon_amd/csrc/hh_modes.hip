@@ -21,7 +21,14 @@
 //    reciprocal form), with contraction off, so they are bit-identical to
 //    the reference's for real operators, and the device runs the
 //    reference's substitution (2D_Program/solver_functions/solver.c:71-86,
-//    3D_Program/solver_functions/solver.c:356-369) in sequential order.
+//    3D_Program/solver_functions/solver.c:356-369) in sequential order with
+//    the same operations, so these modes differ from the reference's only
+//    through their input.  (An exact double-double substitution, evaluated
+//    as a parallel scan, was measured too: more accurate than the
+//    reference, it disagreed with it by up to 1.3e-12 on a manufactured
+//    solution at N = 511, k = 39, where the reference's own FP64
+//    substitution loses 4e-13 to pivots of 4e-3; the sequential order stays
+//    within 3e-14 there.)
 //
 // A near-resonant mode whose estimated condition number exceeds
 // HH_SINGULAR_KAPPA (or whose pivot is zero or not finite) makes its
@@ -70,9 +77,11 @@ inline RowBands row_bands(int dim, const cd *row, double lx, double ly, double l
 }
 
 // Real-operator factors, reference order, FP64 with no fused operations.
-// out rows: (L_q, R_q, Up_q) with R_q the multiplier of the back-substitution
-// (2D: 1/U_q, the reference divides by U_q; 3D: U_q, the reference stores the
-// reciprocal pivot).  Returns false when a pivot is zero or not finite.
+// out rows: (L_q, P_q, R_q, Up_q).  2D: the reference divides by its pivot
+// U_q; P_q = U_q and R_q = RN(1/U_q), with which the device reproduces the
+// correctly rounded quotient (one fused correction step, Markstein).  3D:
+// the reference multiplies by its stored reciprocal pivot U_q; R_q = U_q and
+// P_q = 1/U_q (unused).  Returns false when a pivot is zero or not finite.
 bool factors_real(int dim, int n, const cd *tab, int nc, double lx, double ly, double lxy, double *out) {
     bool ok = true;
     double Uprev = 0, Upprev = 0;
@@ -91,25 +100,28 @@ bool factors_real(int dim, int n, const cd *tab, int nc, double lx, double ly, d
             sup = e(row + 4);
             sub = e(row + 8);
         }
-        double L, U, R;
+        double L, U, R, P;
         if (dim == 2) {
             // LU.c:32-40: L = sub / U_{q-1}, U = dia - L Up_{q-1}
             L = q == 0 ? 0.0 : sub / Uprev;
             U = q == 0 ? dia : dia - L * Upprev;
             R = 1.0 / U;
+            P = U;
             Uprev = U;
         } else {
             // LU.c:61-72: L = sub U_{q-1} (U reciprocal), U = 1 / (dia - Up_{q-1} L)
             L = q == 0 ? 0.0 : sub * Uprev;
             U = q == 0 ? 1.0 / dia : 1.0 / (dia - Upprev * L);
             R = U;
+            P = 1.0 / U;
             Uprev = U;
         }
-        if (!std::isfinite(L) || !std::isfinite(R) || U == 0.0) ok = false;
+        if (!std::isfinite(L) || !std::isfinite(R) || !std::isfinite(P) || U == 0.0) ok = false;
         Upprev = sup;
-        out[3 * q] = L;
-        out[3 * q + 1] = R;
-        out[3 * q + 2] = q == n - 1 ? 0.0 : sup;   // the last row has no superdiagonal
+        out[4 * q] = L;
+        out[4 * q + 1] = P;
+        out[4 * q + 2] = R;
+        out[4 * q + 3] = q == n - 1 ? 0.0 : sup;   // the last row has no superdiagonal
     }
     return ok;
 }
@@ -119,29 +131,32 @@ bool factors_complex(int dim, int n, const cd *tab, int nc, double lx, double ly
     cd Uprev = 0, Upprev = 0;
     for (int q = 0; q < n; q++) {
         const RowBands b = row_bands(dim, tab + (long)q * nc, lx, ly, lxy);
-        cd L, U, R;
+        cd L, U, R, P;
         if (dim == 2) {
             L = q == 0 ? cd(0) : b.sub / Uprev;
             U = q == 0 ? b.dia : b.dia - L * Upprev;
             R = cd(1) / U;
+            P = U;
         } else {
             L = q == 0 ? cd(0) : b.sub * Uprev;
             U = q == 0 ? cd(1) / b.dia : cd(1) / (b.dia - Upprev * L);
             R = U;
+            P = cd(1) / U;
         }
         Uprev = U;
         if (!std::isfinite(std::abs(L)) || !std::isfinite(std::abs(R)) || U == cd(0)) ok = false;
         Upprev = b.sup;
-        out[3 * q] = L;
-        out[3 * q + 1] = R;
-        out[3 * q + 2] = q == n - 1 ? cd(0) : b.sup;
+        out[4 * q] = L;
+        out[4 * q + 1] = P;
+        out[4 * q + 2] = R;
+        out[4 * q + 3] = q == n - 1 ? cd(0) : b.sup;
     }
     return ok;
 }
 #pragma clang fp contract(on)
 
 // ||T^{-1} e||_inf for e = all ones and e = a fixed +-1 pattern, with the
-// factors (L, R, Up) of the reference substitution.
+// factors (L, P, R, Up) of the reference substitution.
 template <typename V> double inv_norm_est(int n, const V *f) {
     double best = 0;
     std::vector<V> y(n);
@@ -150,12 +165,12 @@ template <typename V> double inv_norm_est(int n, const V *f) {
         for (int q = 0; q < n; q++) {
             double e = 1.0;
             if (pass) { s = s * 1664525u + 1013904223u; e = (s >> 31) ? 1.0 : -1.0; }
-            y[q] = q == 0 ? V(e) : V(e) - f[3 * q] * y[q - 1];
+            y[q] = q == 0 ? V(e) : V(e) - f[4 * q] * y[q - 1];
         }
         V x = 0;
         double mx = 0;
         for (int q = n - 1; q >= 0; q--) {
-            x = q == n - 1 ? y[q] * f[3 * q + 1] : (y[q] - f[3 * q + 2] * x) * f[3 * q + 1];
+            x = q == n - 1 ? y[q] * f[4 * q + 2] : (y[q] - f[4 * q + 3] * x) * f[4 * q + 2];
             mx = std::max(mx, std::abs(x));
         }
         best = std::max(best, mx);
@@ -210,12 +225,12 @@ void classify_modes(int dim, int Nx, int Ny, int Nz, int nop, const cd *hcoef, c
             const long e = (long)ms.fmode.size();
             ms.fmode.push_back((int)m);
             ms.bits[(size_t)o * ms.words + m / 32] |= 1u << (m % 32);
-            ms.facc.resize((size_t)(e + 1) * n * 3);
-            cd *fc = ms.facc.data() + (size_t)e * n * 3;
+            ms.facc.resize((size_t)(e + 1) * n * 4);
+            cd *fc = ms.facc.data() + (size_t)e * n * 4;
             bool ok;
             double inv;
             if (real) {
-                std::vector<double> fr(3 * (size_t)n);
+                std::vector<double> fr(4 * (size_t)n);
                 ok = factors_real(dim, n, tab, nc, lx, ly, lxy, fr.data());
                 for (size_t i = 0; i < fr.size(); i++) fc[i] = fr[i];
                 ms.facr.insert(ms.facr.end(), fr.begin(), fr.end());

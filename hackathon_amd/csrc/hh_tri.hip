@@ -467,186 +467,130 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
 
 // Near-resonant modes: the reference's substitution (2D_Program/
 // solver_functions/solver.c:71-86, 3D_Program/solver_functions/
-// solver.c:356-369) with the factors the host computed in the order of its
-// LU() (L_q, R_q, Up_q; the back-substitution multiplies by R_q):
-//   y_0 = d_0,  y_q = d_q - L_q y_{q-1};   x_{n-1} = R y_{n-1},
-//   x_q = (y_q - Up_q x_{q+1}) R_q,
-// evaluated in double-double arithmetic (about 106 bits), so the result is
-// the exact substitution with the reference's pivots rounded once to FP64:
-// it differs from the reference's sequential FP64 substitution only by that
-// substitution's own rounding.  Both first-order recurrences are linear, so
-// they run in parallel: thread t of the block holds R consecutive rows,
-// runs its rows' recurrence from a zero start while carrying the product of
-// its multipliers (y_q = y0_q + g_q y_{s-1}), the chunks' affine maps
-// z -> y0_e + g_e z are combined by an inclusive scan over the block (wave
-// shuffles, then the wave totals through LDS), and each row is fixed up
-// with the value that enters its chunk; the backward pass scans from the
-// other end.  Block (x, y): entry foff[o] + x of operator o = b % nop,
-// right-hand side b = b0 + y, 64 NW <= 512 threads.
-struct DD {
-    double hi, lo;
-};
-HH_DEV DD dd(double a) { return DD{a, 0.0}; }
-HH_DEV DD two_sum(double a, double b) {
-    const double s = a + b, bb = s - a;
-    return DD{s, (a - (s - bb)) + (b - bb)};
-}
-HH_DEV DD quick(double s, double e) {
-    const double h = s + e;
-    return DD{h, e - (h - s)};
-}
-HH_DEV DD add(DD a, DD b) {
-    const DD s = two_sum(a.hi, b.hi);
-    return quick(s.hi, s.lo + (a.lo + b.lo));
-}
-HH_DEV DD neg(DD a) { return DD{-a.hi, -a.lo}; }
-HH_DEV DD mul(DD a, double b) {
-    const double p = a.hi * b;
-    const double e = __builtin_fma(a.hi, b, -p);
-    return quick(p, e + a.lo * b);
-}
-HH_DEV DD mul(DD a, DD b) {
-    const double p = a.hi * b.hi;
-    const double e = __builtin_fma(a.hi, b.hi, -p);
-    return quick(p, e + (a.hi * b.lo + a.lo * b.hi));
-}
-struct CDD {
-    DD re, im;
-};
-HH_DEV CDD cdd(double2 v) { return CDD{dd(v.x), dd(v.y)}; }
-HH_DEV CDD add(CDD a, CDD b) { return CDD{add(a.re, b.re), add(a.im, b.im)}; }
-HH_DEV CDD sub(CDD a, CDD b) { return CDD{add(a.re, neg(b.re)), add(a.im, neg(b.im))}; }
-HH_DEV CDD mul(CDD a, double s) { return CDD{mul(a.re, s), mul(a.im, s)}; }
-HH_DEV CDD mul(CDD a, double2 s) {
-    return CDD{add(mul(a.re, s.x), neg(mul(a.im, s.y))), add(mul(a.re, s.y), mul(a.im, s.x))};
-}
-HH_DEV CDD mul(CDD a, DD s) { return CDD{mul(a.re, s), mul(a.im, s)}; }
-HH_DEV CDD mul(CDD a, CDD s) {
-    return CDD{add(mul(a.re, s.re), neg(mul(a.im, s.im))), add(mul(a.re, s.im), mul(a.im, s.re))};
-}
-// multipliers: real (real operator) or complex
-template <typename OT> struct Mult;
-template <> struct Mult<double> {
-    using M = DD;
-    static HH_DEV M one() { return dd(1.0); }
-    static HH_DEV M zero() { return dd(0.0); }
-    static HH_DEV M scale(M g, double f) { return mul(g, -f); }   // -f g
-};
-template <> struct Mult<double2> {
-    using M = CDD;
-    static HH_DEV M one() { return CDD{dd(1.0), dd(0.0)}; }
-    static HH_DEV M zero() { return CDD{dd(0.0), dd(0.0)}; }
-    static HH_DEV M scale(M g, double2 f) { return mul(g, mk<double>(-f.x, -f.y)); }
-};
-HH_DEV double shfl_d(double v, int src) { return __shfl(v, src, 64); }
-HH_DEV DD shfl(DD v, int src) { return DD{shfl_d(v.hi, src), shfl_d(v.lo, src)}; }
-HH_DEV CDD shfl(CDD v, int src) { return CDD{shfl(v.re, src), shfl(v.im, src)}; }
-
-// affine map z -> B + A z
-template <typename M> struct Aff {
-    CDD B;
-    M A;
-};
-template <typename M> HH_DEV Aff<M> compose(const Aff<M> &f, const Aff<M> &g) {   // f o g
-    return Aff<M>{add(f.B, mul(g.B, f.A)), mul(f.A, g.A)};
-}
-template <typename M> HH_DEV Aff<M> shfl(const Aff<M> &f, int src) { return Aff<M>{shfl(f.B, src), shfl(f.A, src)}; }
-
-// Value entering each thread's chunk: forward (REV = false) the composition
-// of the maps of threads 0..t-1 applied to 0, backward that of threads
-// t+1..NT-1.  All threads of the block call it (two barriers).
-template <bool REV, typename M>
-HH_DEV CDD scan_in(Aff<M> f, Aff<M> *lds, int lane, int wave, int NW) {
-    // inclusive scan inside the wave
-    for (int k = 1; k < 64; k <<= 1) {
-        const int src = REV ? lane + k : lane - k;
-        const Aff<M> g = shfl(f, src < 0 || src > 63 ? lane : src);
-        if (REV ? lane + k < 64 : lane >= k) f = compose(f, g);
+// solver.c:356-369) in its own operation order, with the factors the host
+// computed in the order of its LU():
+//   y_0 = d_0,  y_q = d_q - L_q y_{q-1};
+//   2D: x_q = (y_q - Up_q x_{q+1}) / U_q;   3D: x_q = (y_q - Up_q x_{q+1}) U_q,
+// FP64 with no fused operations, whatever the storage type, so that for
+// real operators every operation rounds as the reference's does and the
+// result differs from the reference's only through its input (the
+// transforms' rounding).  The 2D quotient is the correctly rounded one:
+// q0 = a r, q = q0 + (a - p q0) r with r = RN(1/p) from the host and the
+// residual exact by a fused multiply-add (Markstein).
+//
+// The recurrences are sequential, so the parallelism is across chains (one
+// chain = one near-resonant mode of one right-hand side): a block runs C
+// chains side by side, lane l holding chain l % C and, at position l / C
+// of its wave, R consecutive rows in registers (all loads issued up
+// front).  The chains' positions are walked in order: the C lanes of one
+// position run their rows while the others wait, then hand the carried
+// value to the next position by a lane shuffle, and from wave to wave
+// through LDS.  Block x runs items x C .. x C + C - 1, item = b nfx + e_o,
+// right-hand side b, its operator's near-resonant entry e_o.
+template <typename OT> struct SeqOps;
+template <> struct SeqOps<double> {   // real factors, complex right-hand sides
+    static HH_DEV double2 fms(double a, double2 x, double2 y) {   // y - a x
+#pragma clang fp contract(off)
+        return mk<double>(y.x - a * x.x, y.y - a * x.y);
     }
-    const int last = REV ? 0 : 63;
-    if (lane == last) lds[wave] = f;
-    __syncthreads();
-    // composition of the earlier waves' totals (in processing order)
-    Aff<M> pre = f;   // set below when an earlier wave exists
-    bool have = false;
-    if (!REV) {
-        for (int w = 0; w < wave; w++) { pre = have ? compose(lds[w], pre) : lds[w]; have = true; }
-    } else {
-        for (int w = NW - 1; w > wave; w--) { pre = have ? compose(lds[w], pre) : lds[w]; have = true; }
+    static HH_DEV double2 mul(double a, double2 x) { return mk<double>(a * x.x, a * x.y); }
+    static HH_DEV double q1(double a, double p, double r) {
+        const double q0 = a * r;
+        return __builtin_fma(__builtin_fma(-p, q0, a), r, q0);
     }
-    __syncthreads();
-    // exclusive value: the previous thread's inclusive map applied to the
-    // earlier waves' total, applied to 0
-    const int src = REV ? lane + 1 : lane - 1;
-    const Aff<M> prev = shfl(f, src < 0 || src > 63 ? lane : src);
-    const bool edge = REV ? lane == 63 : lane == 0;
-    const CDD zero{dd(0.0), dd(0.0)};
-    const CDD base = have ? pre.B : zero;
-    if (edge) return base;
-    return add(prev.B, mul(base, prev.A));
-}
+    static HH_DEV double2 div(double2 a, double p, double r) { return mk<double>(q1(a.x, p, r), q1(a.y, p, r)); }
+};
+template <> struct SeqOps<double2> {   // complex factors: C99 products, quotients through the reciprocal
+    static HH_DEV double2 mul(double2 a, double2 x) {
+#pragma clang fp contract(off)
+        return mk<double>(a.x * x.x - a.y * x.y, a.x * x.y + a.y * x.x);
+    }
+    static HH_DEV double2 fms(double2 a, double2 x, double2 y) {
+#pragma clang fp contract(off)
+        const double2 p = mul(a, x);
+        return mk<double>(y.x - p.x, y.y - p.y);
+    }
+    static HH_DEV double2 div(double2 a, double2, double2 r) { return mul(r, a); }
+};
 
-template <int R, class Prob, typename OT>
+HH_DEV double2 shfl_up2(double2 v, int k) { return mk<double>(__shfl_up(v.x, k, 64), __shfl_up(v.y, k, 64)); }
+HH_DEV double2 shfl_down2(double2 v, int k) { return mk<double>(__shfl_down(v.x, k, 64), __shfl_down(v.y, k, 64)); }
+
+template <int R, int C, bool DIV, class Prob, typename OT>
 __global__ __launch_bounds__(512) void k_tri_seq(Prob prob, const int *foff, const int *fmode, const OT *fac,
-                                                 int nop, int b0) {
-    using MT = Mult<OT>;
-    using M = typename MT::M;
-    __shared__ Aff<M> lds[8];
+                                                 long nent, int nfx, int batch, int nop) {
+    using S = SeqOps<OT>;
+    constexpr int PW = 64 / C;   // positions per wave
+    __shared__ double2 carry[C];
     const int n = prob.n();
-    const int b = b0 + (int)blockIdx.y;
-    const int o = b % nop;
-    const int cnt = foff[o + 1] - foff[o];
-    if ((int)blockIdx.x >= cnt) return;   // uniform over the block
-    const int e = foff[o] + (int)blockIdx.x;
-    const auto md = prob.mode(fmode[e], b);
-    const OT *F = fac + (long)e * n * 3;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6, NW = blockDim.x >> 6;
-    const int s = t * R;
-    CDD y[R];
-    M g[R];
-    OT Lf[R], Rf[R], Uf[R];
+    const int c = lane % C, pw = lane / C;
+    const int s = (wave * PW + pw) * R;   // first row of this lane
+    const long item = (long)blockIdx.x * C + c;
+    const int b = (int)(item / nfx), x = (int)(item % nfx);
+    const int o = b < batch ? b % nop : 0;
+    const bool live = b < batch && x < foff[o + 1] - foff[o];
+    // idle chains of the last block run on a valid item and store nothing
+    const int bb = live ? b : 0, e = live ? foff[o] + x : 0;
+    const auto md = prob.mode(fmode[e], bb);
+    double2 d[R];
+    OT L[R], Pv[R], Rv[R], Up[R];
 #pragma unroll
     for (int i = 0; i < R; i++) {
         const int q = min(s + i, n - 1);
-        y[i] = cdd(prob.rhs64(b, md, q));
-        Lf[i] = F[3 * q];
-        Rf[i] = F[3 * q + 1];
-        Uf[i] = F[3 * q + 2];
+        d[i] = prob.rhs64(bb, md, q);
+        const OT *f = fac + ((long)q * nent + e) * 4;
+        L[i] = f[0];
+        Pv[i] = f[1];
+        Rv[i] = f[2];
+        Up[i] = f[3];
     }
-    // forward, local: y0 from a zero start, g the product of -L
-    CDD yc{dd(0.0), dd(0.0)};
-    M gc = MT::one();
+    // forward: y_q = d_q - L_q y_{q-1} (L_0 = 0)
+    double2 cy = mk<double>(0, 0);
+    for (int w = 0; w < NW; w++) {
+        if (wave == w) {
+            if (w > 0) cy = carry[c];
+            for (int pp = 0; pp < PW; pp++) {
+                if (pw == pp) {
 #pragma unroll
-    for (int i = 0; i < R; i++) {
-        if (s + i < n) {
-            yc = sub(y[i], mul(yc, Lf[i]));   // L_0 = 0 starts the recurrence at row 0
-            gc = MT::scale(gc, Lf[i]);
+                    for (int i = 0; i < R; i++)
+                        if (s + i < n) {
+                            d[i] = S::fms(L[i], cy, d[i]);
+                            cy = d[i];
+                        }
+                    if (pp == PW - 1) carry[c] = cy;
+                }
+                if (pp < PW - 1) cy = shfl_up2(cy, C);
+            }
         }
-        y[i] = yc;
-        g[i] = gc;
+        __syncthreads();
     }
-    CDD yin = scan_in<false, M>(Aff<M>{yc, gc}, lds, lane, wave, NW);
+    // backward: x_q = (y_q - Up_q x_{q+1}) / U_q (2D) or * U_q (3D) (Up_{n-1} = 0)
+    cy = mk<double>(0, 0);
+    for (int w = NW - 1; w >= 0; w--) {
+        if (wave == w) {
+            if (w < NW - 1) cy = carry[c];
+            for (int pp = PW - 1; pp >= 0; pp--) {
+                if (pw == pp) {
 #pragma unroll
-    for (int i = 0; i < R; i++) y[i] = add(y[i], mul(yin, g[i]));
-    // backward, local: x0 from a zero start at the chunk's end, h the product of -R Up
-    CDD xc{dd(0.0), dd(0.0)};
-    M hc = MT::one();
-#pragma unroll
-    for (int i = R - 1; i >= 0; i--) {
-        if (s + i < n) {
-            xc = mul(sub(y[i], mul(xc, Uf[i])), Rf[i]);   // Up_{n-1} = 0 starts it at the last row
-            hc = mul(MT::scale(hc, Uf[i]), Rf[i]);
+                    for (int i = R - 1; i >= 0; i--)
+                        if (s + i < n) {
+                            const double2 a = S::fms(Up[i], cy, d[i]);
+                            if constexpr (DIV) d[i] = S::div(a, Pv[i], Rv[i]);
+                            else d[i] = S::mul(Rv[i], a);
+                            cy = d[i];
+                        }
+                    if (pp == 0) carry[c] = cy;
+                }
+                if (pp > 0) cy = shfl_down2(cy, C);
+            }
         }
-        y[i] = xc;
-        g[i] = hc;
+        __syncthreads();
     }
-    CDD xin = scan_in<true, M>(Aff<M>{xc, hc}, lds, lane, wave, NW);
+    if (!live) return;
 #pragma unroll
     for (int i = 0; i < R; i++)
-        if (s + i < n) {
-            const CDD x = add(y[i], mul(xin, g[i]));
-            prob.store64(b, md, s + i, mk<double>(x.re.hi + x.re.lo, x.im.hi + x.im.lo));
-        }
+        if (s + i < n) prob.store64(b, md, s + i, d[i]);
 }
 
 // LDS bytes of the reduced system: two buffers of A, C (operator values of
@@ -776,8 +720,8 @@ static hipError_t launch_tri_mm(const TriHelm &t, const void *in, void *out, hip
     }
 }
 
-template <int R, typename Ti, typename To, int DIM, bool REAL>
-static hipError_t launch_seq_r(const TriHelm &t, const void *in, void *out, hipStream_t st) {
+template <int R, int C, typename Ti, typename To, int DIM, bool REAL>
+static hipError_t launch_seq_rc(const TriHelm &t, const void *in, void *out, hipStream_t st) {
     using OT = typename std::conditional<REAL, double, double2>::type;
     HelmProb<double, Ti, To, DIM, REAL> pb;
     pb.bands.Nx = t.Nx;
@@ -795,31 +739,41 @@ static hipError_t launch_seq_r(const TriHelm &t, const void *in, void *out, hipS
     pb.fbits = nullptr;
     pb.fwords = 0;
     const int n = pb.bands.n;
-    const int NW = (n + 64 * R - 1) / (64 * R);
-    if (NW > 8) return hipErrorInvalidValue;
-    const int nop = pb.bands.nop;
-    for (int b0 = 0; b0 < t.batch; b0 += 65535) {
-        const int nb = t.batch - b0 < 65535 ? t.batch - b0 : 65535;
-        hipLaunchKernelGGL((k_tri_seq<R, HelmProb<double, Ti, To, DIM, REAL>, OT>), dim3((unsigned)t.nfx, (unsigned)nb),
-                           dim3(64 * NW), 0, st, pb, t.foff, t.fmode, (const OT *)t.fac, nop, b0);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
+    constexpr int PW = 64 / C;
+    const int G = ((n + R - 1) / R + PW - 1) / PW * PW;   // positions, whole waves
+    const int NT = G * C;
+    if (NT > 512) return hipErrorInvalidValue;
+    const long items = (long)t.nfx * t.batch;
+    const long blocks = (items + C - 1) / C;
+    if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_tri_seq<R, C, DIM == 2, HelmProb<double, Ti, To, DIM, REAL>, OT>), dim3((unsigned)blocks),
+                       dim3(NT), 0, st, pb, t.foff, t.fmode, (const OT *)t.fac, t.nent, t.nfx, t.batch, pb.bands.nop);
+    return hipGetLastError();
 }
 
+// As many chains per block as 512 threads hold with R rows per lane.
+template <int R, typename Ti, typename To, int DIM, bool REAL>
+static hipError_t launch_seq_R(const TriHelm &t, const void *in, void *out, hipStream_t st) {
+    const int n = t.dim == 2 ? t.Ny : t.Nz;
+    const int g = (n + R - 1) / R;   // positions of one chain
+    auto fits = [&](int C) { const int PW = 64 / C; return ((g + PW - 1) / PW * PW) * C <= 512; };
+    if (fits(64)) return launch_seq_rc<R, 64, Ti, To, DIM, REAL>(t, in, out, st);
+    if (fits(32)) return launch_seq_rc<R, 32, Ti, To, DIM, REAL>(t, in, out, st);
+    if (fits(16)) return launch_seq_rc<R, 16, Ti, To, DIM, REAL>(t, in, out, st);
+    if (fits(8)) return launch_seq_rc<R, 8, Ti, To, DIM, REAL>(t, in, out, st);
+    if (fits(4)) return launch_seq_rc<R, 4, Ti, To, DIM, REAL>(t, in, out, st);
+    if (fits(2)) return launch_seq_rc<R, 2, Ti, To, DIM, REAL>(t, in, out, st);
+    if (fits(1)) return launch_seq_rc<R, 1, Ti, To, DIM, REAL>(t, in, out, st);
+    return hipErrorInvalidValue;
+}
+
+// 16 rows per lane with real factors; 8 with complex ones (twice the
+// registers), except above n = 4096 where 512 threads need 16
 template <typename Ti, typename To, int DIM, bool REAL>
 static hipError_t launch_seq_d(const TriHelm &t, const void *in, void *out, hipStream_t st) {
-    // The double-double scan over the block costs about 1000 FP64 operations per
-    // thread against about 90 per row of the local recurrences, so blocks of
-    // about 128 threads (8 rows each at n = 1023) keep the work low while the
-    // critical path stays a few thousand cycles.
     const int n = t.dim == 2 ? t.Ny : t.Nz;
-    if (n <= 128) return launch_seq_r<1, Ti, To, DIM, REAL>(t, in, out, st);
-    if (n <= 256) return launch_seq_r<2, Ti, To, DIM, REAL>(t, in, out, st);
-    if (n <= 512) return launch_seq_r<4, Ti, To, DIM, REAL>(t, in, out, st);
-    if (n <= 4096) return launch_seq_r<8, Ti, To, DIM, REAL>(t, in, out, st);
-    return launch_seq_r<16, Ti, To, DIM, REAL>(t, in, out, st);
+    if (REAL || n > 4096) return launch_seq_R<16, Ti, To, DIM, REAL>(t, in, out, st);
+    return launch_seq_R<8, Ti, To, DIM, REAL>(t, in, out, st);
 }
 
 template <typename Ti, typename To>
