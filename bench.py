@@ -46,6 +46,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the oracle comparison (profiling runs)")
+    ap.add_argument("--no-sections", action="store_true", help="headline only: no C1 / C3 sections")
     ap.add_argument("--check-rhs", type=int, default=2, help="right-hand sides per order compared with the oracle")
     ap.add_argument("--workload", default="c2", choices=["c2", "3d", "uneven", "serve"],
                     help="c2: the headline 2D batch (default); 3d: 255^3 FP64 and mixed precision; uneven: 512 "
@@ -120,6 +121,122 @@ def cpu_baseline(O, N, coef, F_host, seconds, threads=None):
             "sample": f"oracle solver() (own FFT DST, plan creation inside the call, "
                       f"{'serial' if threads == 1 else 'OpenMP over rows/modes'}) on "
                       f"{done} 6th-order right-hand sides of the same N={N} k=50 workload, {t_used:.1f} s"}
+
+
+def cpu_steady(O, N, coef, F_host, seconds, threads=None):
+    """The same solver() with its DST plan created once outside the timed
+    calls (steady state, plans reused)."""
+    threads = host_threads() if threads is None else threads
+    LUU = O.lu2d(N, N, coef, threads)
+    plan = O.DstPlan(N)
+    done, t_used = 0, 0.0
+    while t_used < seconds:
+        t0 = time.perf_counter()
+        O.solver2d_planned(plan, N, N, F_host[done % F_host.shape[0]], LUU, threads)
+        t_used += time.perf_counter() - t0
+        done += 1
+    return {"value": done * N * N / t_used, "unit": "complex DOF/s", "cores": threads, "kind": "port",
+            "sample": f"oracle solver() with its DST plan reused across calls (steady state), "
+                      f"{'serial' if threads == 1 else 'OpenMP over rows/modes'}, {done} right-hand sides of "
+                      f"the same N={N} workload, {t_used:.1f} s"}
+
+
+def _events(torch, n):
+    return [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(n)]
+
+
+def run_c1(torch, H, O, args):
+    """C1: 2D order 6, N = 100 (N+1 = 101: the dense DST path), k = 10,
+    u = e^{i pi/4} sin(3 pi x) sin(2 pi y) (the configuration's (1+i) up to the
+    factor sqrt 2, which changes no relative figure).  GPU latency of one
+    solve, the oracle check, and the reference's own CPU configuration
+    (./Direct_Solver 6 100 2: two OpenMP threads) as it runs (DST plans
+    created inside the call) and in steady state."""
+    N, k, order = 100, 10.0, 6
+    s = O.solve2d(order, N, O.SINE, [k, 0.0, 3, 2, math.pi / 4], threads=2)
+    ctx = H.Context(0)
+    plan = H.Plan2D(ctx, N, N, s["coef"])
+    rhs = torch.from_numpy(s["rhs"]).cuda()
+    out = torch.empty_like(rhs)
+    for _ in range(5):
+        plan.solve(rhs, out=out)
+    torch.cuda.synchronize()
+    ev = _events(torch, 50)
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record()
+        plan.solve(rhs, out=out)
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = [a.elapsed_time(b) for a, b in ev]
+    g = out.cpu().numpy()
+    res = {"workload": "2D order 6, N=100, k=10, u=e^{i pi/4} sin(3 pi x) sin(2 pi y), one right-hand side",
+           "gpu_dof_per_s": N * N * len(ev) / wall, "p50_ms": float(np.percentile(lat, 50)),
+           "p99_ms": float(np.percentile(lat, 99)),
+           "max_rel_err_vs_oracle": float(np.abs(g - s["sol"]).max() / np.abs(s["sol"]).max()),
+           "disc_err_vs_exact": float(np.abs(g - s["u"]).max() / np.abs(s["u"]).max())}
+    if not args.no_cpu:
+        F = s["rhs"][None, :]
+        cpu = cpu_baseline(O, N, s["coef"], F, 2.0, threads=2)
+        cpu["sample"] = cpu["sample"].replace(f"6th-order right-hand sides of the same N={N} k=50 workload",
+                                              "solves of the C1 right-hand side")
+        res["cpu_baseline_2threads"] = cpu
+        res["cpu_baseline_2threads_steady"] = cpu_steady(O, N, s["coef"], F, 2.0, threads=2)
+    return res
+
+
+def run_c3_brief(torch, H, O, args):
+    """C3: 3D order 6, 255^3, k = 30, u = (1+i) sin(a pi x) sin(b pi y) sin(c pi z)
+    with (a, b, c) drawn from the seed: FP64 and mixed precision (two fixed
+    sweeps) on the GPU, the oracle check, and the reference's 3D CPU solver
+    (serial, as it ships) beside the OpenMP restatement."""
+    N, k, order = 255, 30.0, 6
+    rng = np.random.default_rng((args.seed, 3))
+    a, b, c = (int(v) for v in rng.integers(1, 21, size=3))
+    s = O.setup3d(order, N, N, N, O.SINE, [k, 0.0, a, b, c, 1.0, 1.0], threads=host_threads())
+    LUU = O.lu3d(N, N, N, s["coef"])
+    ref = O.solver3d(N, N, N, s["rhs"], LUU, host_threads())
+    ctx = H.Context(0)
+    rhs = torch.from_numpy(s["rhs"]).cuda()
+    out = torch.empty_like(rhs)
+    res = {"workload": f"3D order 6, N=255^3, k=30, u=(1+i) sin(a pi x) sin(b pi y) sin(c pi z), "
+                       f"(a,b,c)=({a},{b},{c}) from seed ({args.seed}, 3)"}
+    for name, prec in (("fp64", H.PREC_F64), ("mixed_2_sweeps", H.PREC_MIXED)):
+        plan = H.Plan3D(ctx, N, N, N, s["coef"].reshape(12, N), precision=prec)
+        if prec == H.PREC_MIXED:
+            plan.set_refine(0.0, 2)
+        for _ in range(2):
+            plan.solve(rhs, out=out)
+        torch.cuda.synchronize()
+        ev = _events(torch, 10)
+        t0 = time.perf_counter()
+        for e0, e1 in ev:
+            e0.record()
+            plan.solve(rhs, out=out)
+            e1.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lat = [e0.elapsed_time(e1) for e0, e1 in ev]
+        g = out.cpu().numpy()
+        res[name] = {"dof_per_s": N ** 3 * len(ev) / wall, "p50_ms": float(np.percentile(lat, 50)),
+                     "max_rel_err_vs_oracle": float(np.abs(g - ref).max() / np.abs(ref).max())}
+        plan.close()
+    res["mixed_over_fp64"] = res["mixed_2_sweeps"]["dof_per_s"] / res["fp64"]["dof_per_s"]
+    if not args.no_cpu:
+        def sample(nthreads, seconds):
+            done, t_used = 0, 0.0
+            while t_used < seconds:
+                t0 = time.perf_counter()
+                O.solver3d(N, N, N, s["rhs"], LUU, nthreads)
+                t_used += time.perf_counter() - t0
+                done += 1
+            how = "serial, as the reference's 3D CPU solver ships" if nthreads == 1 else "OpenMP over layers/modes"
+            return {"value": done * N ** 3 / t_used, "unit": "complex DOF/s", "cores": nthreads, "kind": "port",
+                    "sample": f"oracle 3D solver() ({how}), {done} solves of the C3 right-hand side, {t_used:.1f} s"}
+        res["cpu_baseline"] = sample(host_threads(), 3.0)
+        res["cpu_baseline_serial"] = sample(1, 1.0)
+    return res
 
 
 # ---------------------------------------------------------------- other workloads
@@ -556,6 +673,7 @@ def main():
     check = {}
     cpu = None
     cpu_serial = None
+    cpu_steady16 = None
     if not (args.no_check and args.no_cpu):
         from oracle import oracle as O
         for o in orders:
@@ -572,10 +690,20 @@ def main():
                     derr.append(float(np.abs(g - u).max() / np.abs(u).max()))
                 check[f"order{o}"] = {"max_rel_err_vs_oracle": max(errs), "disc_err_vs_exact": max(derr),
                                       "rhs_checked": len(errs)}
+                # white noise puts energy in the nearly singular low modes,
+                # which the manufactured sines leave out
+                wn = torch.randn(N * N, dtype=torch.complex128, device=dev,
+                                 generator=torch.Generator(device=dev).manual_seed(args.seed + o))
+                wsol = plans[o].solve(wn).cpu().numpy()
+                wref = O.solver2d(N, N, wn.cpu().numpy(), LUU, 16)
+                check[f"order{o}"]["white_noise_rel_err_vs_oracle"] = float(
+                    np.abs(wsol - wref).max() / np.abs(wref).max())
+                check[f"order{o}"]["near_resonant_modes"] = plans[o].mode_info()[0]
             if not args.no_cpu and o == orders[-1]:
                 Fall = F[o].cpu().numpy().reshape(B, N * N)
                 cpu = cpu_baseline(O, N, coef, Fall, args.cpu_seconds)
                 cpu_serial = cpu_baseline(O, N, coef, Fall, args.cpu_seconds / 2, threads=1)
+                cpu_steady16 = cpu_steady(O, N, coef, Fall, args.cpu_seconds / 2)
 
     out = {
         "metric": "complex DOF/s at <=1e-10 rel err vs FP64 oracle; p50/p99 solve latency",
@@ -605,8 +733,16 @@ def main():
         "solve_level_frac": value * 32 / (HBM_PEAK_GBS * 1e9),
         "cpu_baseline": cpu,
         "cpu_baseline_serial": cpu_serial,
+        "cpu_baseline_steady": cpu_steady16,
         "host_cpus": host_cpus(),
     }
+    if not args.no_sections:
+        # the other single-GPU configurations, beside the headline
+        from oracle import oracle as O
+        log("C1 section")
+        out["c1"] = run_c1(torch, H, O, args)
+        log("C3 section")
+        out["c3"] = run_c3_brief(torch, H, O, args)
     print(json.dumps(out), flush=True)
     if args.copy_probe > 0:
         n = args.copy_probe * (1 << 20) // 8
