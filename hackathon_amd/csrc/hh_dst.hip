@@ -149,7 +149,7 @@ __global__ __launch_bounds__((dst_cfg<L, TPB>::THREADS)) void k_dst_fft_rows(
     // Branch-free: every load is issued (index clamped into the row) and
     // the zero entries of the odd extension are selected afterwards, so the
     // 16 loads of a thread are all in flight together.
-    auto get = [&](int n) -> C2<T> {
+    auto get = [&](int n, int) -> C2<T> {
         const bool zero = (n == 0) || (n == M);
         const int i = zero ? 0 : (n < M ? n - 1 : L - n - 1);
         C2<T> v;
@@ -222,7 +222,7 @@ __global__ __launch_bounds__((dst_cfg<L, TPB>::THREADS)) void k_dst_fft_cols(
     __syncthreads();
     const int li = threadIdx.x / P, t = threadIdx.x % P;
     T *sre = smem + li * LS, *sim = smem + li * LS + LP;
-    auto get = [&](int n) -> C2<T> {
+    auto get = [&](int n, int) -> C2<T> {
         if (n == 0 || n == M) return mk<T>(0, 0);
         if (n < M) return mk<T>(sre[lds_pad(n)], sim[lds_pad(n)]);
         return mk<T>(-sre[lds_pad(L - n)], -sim[lds_pad(L - n)]);
@@ -258,6 +258,22 @@ template <int M, int TPB = 256> struct half_cfg {
     static constexpr int LP = fft_shape<M>::LP;
     static constexpr int LS = line_stride(LP, LPB);
     static constexpr int K = (M / 2) / P;   // pairs (F_k, F_{M-k}) per thread, M >= 2P
+};
+
+// sin(pi n/M) of the first-stage value n = t + r P of lane t:
+// sin(pi t/M) cos(pi r P/M) + cos(pi t/M) sin(pi r P/M), the lane's pair
+// loaded once and the slot's pair the same in every lane (uniform loads),
+// from sn[j] = sin(pi j/M), j < M, with cos(pi a/M) = +-sn[|M/2 - a|].
+template <typename T, int M> struct HalfSine {
+    const T *sn;
+    T st, ct;
+    HH_DEV HalfSine(const T *s, int t) : sn(s), st(s[t]), ct(s[M / 2 - t]) {}
+    HH_DEV T at(int r) const {
+        const int a = r * fft_shape<M>::P;
+        const T sr = sn[a];
+        const T cr = a <= M / 2 ? sn[M / 2 - a] : -sn[a - M / 2];
+        return st * cr + ct * sr;
+    }
 };
 
 // Post-processing of one line whose transform F sits in (sre, sim); writes the
@@ -341,7 +357,8 @@ __global__ __launch_bounds__((half_cfg<M, TPB>::THREADS)) void k_dst_half_rows(c
     }
     // branch-free (clamped indices, zero selected for n = 0) so that all
     // loads of a thread are in flight together; rows past the end read row 0
-    auto get = [&](int n) -> C2<T> {
+    const HalfSine<T, M> hs(sn, t);
+    auto get = [&](int n, int r) -> C2<T> {
         const int ia = n > 0 ? n - 1 : 0, ib = n > 0 ? M - n - 1 : 0;
         C2<T> a, b;
         if constexpr (TIN) {
@@ -352,9 +369,9 @@ __global__ __launch_bounds__((half_cfg<M, TPB>::THREADS)) void k_dst_half_rows(c
             a = cconv<T, Ti>(x[ia]);
             b = cconv<T, Ti>(x[ib]);
         }
-        const T s = sn[n];
-        const C2<T> r = mk<T>(s * (a.x + b.x) + (T)0.5 * (a.x - b.x), s * (a.y + b.y) + (T)0.5 * (a.y - b.y));
-        return n > 0 ? r : mk<T>(0, 0);
+        const T s = hs.at(r);
+        const C2<T> z = mk<T>(s * (a.x + b.x) + (T)0.5 * (a.x - b.x), s * (a.y + b.y) + (T)0.5 * (a.y - b.y));
+        return n > 0 ? z : mk<T>(0, 0);
     };
     auto put = [&](int k, C2<T> z) {
         sre[lds_pad(k)] = z.x;
@@ -415,12 +432,13 @@ __global__ __launch_bounds__((half_cfg<M, TPB>::THREADS)) void k_dst_half_cols(c
     const int li = threadIdx.x / P, t = threadIdx.x % P;
     T *sre = smem + li * LS, *sim = smem + li * LS + LP;
     const T *sn = reinterpret_cast<const T *>(aux + M);
-    auto get = [&](int n) -> C2<T> {
+    const HalfSine<T, M> hs(sn, t);
+    auto get = [&](int n, int r) -> C2<T> {
         if (n == 0) return mk<T>(0, 0);
         const int nb = M - n;
         const T ar = sre[lds_pad(n)], ai = sim[lds_pad(n)];
         const T br = nb < M ? sre[lds_pad(nb)] : (T)0, bi = nb < M ? sim[lds_pad(nb)] : (T)0;
-        const T s = sn[n];
+        const T s = hs.at(r);
         return mk<T>(s * (ar + br) + (T)0.5 * (ar - br), s * (ai + bi) + (T)0.5 * (ai - bi));
     };
     auto put = [&](int k, C2<T> z) {

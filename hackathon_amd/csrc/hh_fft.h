@@ -97,13 +97,40 @@ template <typename T, int R> HH_DEV void dftR(C2<T>* v) {
     else if constexpr (R == 16) dft16<T>(v);
 }
 
+// Twiddles w[r] = W_{NS R}^{k r}, r < R, of butterfly k from the table
+// tw[q] = e^{-2 pi i q/L}.  Radix 8 and 16 split r = r1 + 4 r2: three loads
+// of W^{k r1}, R/4 - 1 of W^{4 k r2}, and one complex product for each of
+// the other twiddles (one rounding more than a table entry).
+template <typename T, int L, int R, int NS>
+HH_DEV void stage_twiddles(const C2<T>* __restrict__ tw, int k, C2<T>* w) {
+    constexpr int S = L / (NS * R);
+    w[0] = mk<T>(1, 0);
+    if constexpr (R < 8) {
+#pragma unroll
+        for (int r = 1; r < R; r++) w[r] = tw[k * r * S];
+    } else {
+        C2<T> lo[4], hi[R / 4];
+#pragma unroll
+        for (int r1 = 1; r1 < 4; r1++) lo[r1] = tw[k * r1 * S];
+#pragma unroll
+        for (int r2 = 1; r2 < R / 4; r2++) hi[r2] = tw[k * 4 * r2 * S];
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+            const int r1 = r % 4, r2 = r / 4;
+            w[r] = r2 == 0 ? lo[r1] : r1 == 0 ? hi[r2] : cmul<T>(hi[r2], lo[r1]);
+        }
+    }
+}
+
 template <int L> struct fft_shape {
     static constexpr int V = L < 16 ? L : 16;   // values per thread
     static constexpr int P = L / V;             // threads per line
     static constexpr int LP = L + L / 16 + 1;   // padded LDS line length
 };
 
-// One Stockham stage.  FIRST: inputs come from get(n); otherwise from LDS.
+// One Stockham stage.  FIRST: inputs come from get(n, r), r = n / P the
+// value's slot (compile time; the first stage has radix V, so n = t + r P);
+// otherwise from LDS.
 // LAST: outputs go to put(k, z) after an optional barrier (set LASTBAR when
 // put() writes to the LDS line buffer); otherwise they go back to LDS.
 template <typename T, int L, int R, int NS, bool FIRST, bool LAST, bool LASTBAR, class Get, class Put>
@@ -118,17 +145,30 @@ HH_DEV void fft_stage(T* sre, T* sim, const C2<T>* __restrict__ tw, int t, Get& 
 #pragma unroll
         for (int r = 0; r < R; r++) {
             int idx = j + r * (L / R);
-            if constexpr (FIRST) v[b * R + r] = get(idx);
+            if constexpr (FIRST) v[b * R + r] = get(idx, r);
             else v[b * R + r] = mk<T>(sre[lds_pad(idx)], sim[lds_pad(idx)]);
         }
     }
+    // L, NS and P are powers of two: either NS divides P, and every
+    // butterfly j = t + b P of the thread has k = t mod NS, or the thread's
+    // butterflies all lie below NS (NB P = NS), k = t + b P and
+    // W^{k r} = W^{t r} W^{b P r}, the second factor the same in every lane
+    // (a wave-uniform load).  Either way the per-lane twiddles are loaded
+    // once per stage, not once per butterfly.
+    static_assert(NS <= P || NB * P == NS, "unexpected stage shape");
+    C2<T> w[R];
+    if constexpr (NS > 1) stage_twiddles<T, L, R, NS>(tw, NS <= P ? (t & (NS - 1)) : t, w);
 #pragma unroll
     for (int b = 0; b < NB; b++) {
-        int j = t + b * P;
         if constexpr (NS > 1) {
-            int k = j % NS;
 #pragma unroll
-            for (int r = 1; r < R; r++) v[b * R + r] = cmul<T>(v[b * R + r], tw[k * r * (L / (NS * R))]);
+            for (int r = 1; r < R; r++) {
+                C2<T> x = w[r];
+                if constexpr (NS > P) {
+                    if (b > 0) x = cmul<T>(x, tw[b * P * r * (L / (NS * R))]);
+                }
+                v[b * R + r] = cmul<T>(v[b * R + r], x);
+            }
         }
         dftR<T, R>(&v[b * R]);
     }
