@@ -81,7 +81,7 @@ def tune(plan, rhs, out, candidates=CANDIDATES, reps=3, verbose=False, tiles=Non
         if lib.hh_plan_set_tri_shape(plan.h, w, m, r) != 0:
             continue
         for tw in tiles:
-            if tw > 0 and (tw < w or tw % w):
+            if tw > 0 and w > 0 and (tw < w or tw % w):
                 continue
             H._check(lib.hh_plan_set_tile_width(plan.h, tw))
             try:

@@ -1039,18 +1039,90 @@ static void fp32_storage_rows(const hh_plan *p, int n, long nmodes, hh::TriHelm 
     }
 }
 
+// Shape of `rows` rows per thread and about `threads` threads per block:
+// W the largest power of two with W P <= threads, in [wmin, 16], and no
+// more than the modes need.
+static void shape_for(int n, long nmodes, int rows, int threads, int wmin, hh::TriHelm &t) {
+    int P = (n + rows - 1) / rows;
+    if (P > n / 2) P = n / 2;
+    if (P < 1) P = 1;
+    int w = 1;
+    while (w * 2 * P <= threads && w * 2 <= 16) w *= 2;
+    if (w < wmin) w = wmin;
+    while (w > 1 && w / 2 >= nmodes) w /= 2;
+    t.W = w; t.P = P; t.mmax = rows;
+}
+
+// Built-in block shape of the mode solves (W, P, rows and right-hand sides
+// per thread).  kind 0: FP64 elimination on FP64 storage, 1: FP64
+// elimination on FP32 storage, 2: FP32 elimination; per_op: right-hand
+// sides per operator in the batch.  Measured on MI355X (order-6 operators
+// with k^2 = 2500 or 2500 + 50i, row-major spectrum; microseconds of the
+// mode pass of one solve, near-resonant modes included; in brackets the
+// former rule of 256-thread blocks, 16 rows per thread above n = 512):
+//   real bands, FP64
+//     2D  n <= 512: 16 rows, 2 rhs, ~512 threads: 127 x 256: 53.9 (60.0),
+//         255 x 64: 59.2 (66.3), 511 x 64: 184 (216);
+//         n <= 1024: 8 rows, 1 rhs, 512 threads: 1023 x 64: 632 (693)
+//         (two blocks of 112 VGPRs per CU against one of 252);
+//         above: 16 rows, 2 rhs, 512 threads: 2047 x 16: 848 (983)
+//     3D  n <= 128: 8 rows, 2 rhs, 128 threads: 127^3 x 8: 138 (147);
+//         above: 16 rows, 256 threads: 255^3 x 1: 187 (212)
+//   complex bands, FP64
+//         n <= 512: 8 rows, 2 rhs, 256 threads: 2D 127: 67 (75), 255: 79
+//         (86), 511: 240 (265); 3D 127^3 x 8: 169 (228);
+//         above: 16 rows, 1 rhs, W >= 4: 2047 x 16: 1292 (1321)
+//   real bands, 2D, FP32 elimination: 16 rows, 512 threads: 1023 x 64:
+//         449 (506), 2047 x 16: 609 (733), 511: 131 (133), 255: 51 (53)
+// and the former rule elsewhere (within 2% of the best shape measured).
+// Two right-hand sides per thread need two per operator in the batch.
+static void builtin_tri_shape(const hh_plan *p, int kind, long per_op, bool real, hh::TriHelm &t) {
+    const int n = p->dim == 2 ? p->Ny : p->Nz;
+    const long nmodes = p->dim == 2 ? p->Nx : (long)p->Nx * p->Ny;
+    hh::tri_shape(n, nmodes, kind == 0 ? hh::F64 : hh::F32, &t.W, &t.P, &t.mmax);
+    t.nr = 1;
+    if (kind == 1) {
+        hh::tri_shape(n, nmodes, hh::F64, &t.W, &t.P, &t.mmax);
+        fp32_storage_rows(p, n, nmodes, t);
+        // real bands, 16 rows: two right-hand sides per thread share the
+        // operator's pivots (N=1023 x 64: 419 against 531 us per launch)
+        // when the grid still has 8 blocks per CU or more
+        if (real && t.mmax == 16 && per_op >= 2) {
+            const long tiles = (nmodes + t.W - 1) / t.W;
+            if (tiles * p->nop * ((per_op + 1) / 2) >= 2048) t.nr = 2;
+        }
+        return;
+    }
+    if (n <= 16) return;
+    int nr = 1;
+    if (kind == 0 && real && p->dim == 2) {
+        if (n <= 512) { shape_for(n, nmodes, 16, 512, 1, t); nr = 2; }
+        else if (n <= 1024) shape_for(n, nmodes, 8, 512, 1, t);
+        else { shape_for(n, nmodes, 16, 512, 1, t); nr = 2; }
+    } else if (kind == 0 && real) {
+        if (n <= 128) { shape_for(n, nmodes, 8, 128, 1, t); nr = 2; }
+        else { shape_for(n, nmodes, 16, 256, 1, t); nr = 2; }
+    } else if (kind == 0) {
+        if (n <= 512) { shape_for(n, nmodes, 8, 256, 1, t); nr = 2; }
+        else shape_for(n, nmodes, 16, 256, 4, t);
+    } else if (real && p->dim == 2) {
+        shape_for(n, nmodes, 16, 512, 1, t);
+    }
+    const size_t op = real ? 8 : 16;
+    if (nr == 2 && per_op >= 2 && hh::tri_lds_bytes(op, 16, t.P, t.W, 2) <= 160 * 1024) t.nr = 2;
+}
+
 extern "C" int hh_plan_get_tri_shape(const hh_plan *p, int *W, int *P, int *rows_per_thread, int *rhs_per_thread) {
     if (!p) return fail(HH_ERR_ARG, "null plan");
     hh::TriHelm t;
     int n = p->dim == 2 ? p->Ny : p->Nz;
-    long nmodes = p->dim == 2 ? p->Nx : (long)p->Nx * p->Ny;
-    hh::tri_shape(n, nmodes, hh::F64, &t.W, &t.P, &t.mmax);
-    if (p->prec == HH_PREC_MIXED) fp32_storage_rows(p, n, nmodes, t);
-    t.nr = 1;
+    // the shape of the solve's elimination (mixed precision: FP32), for a
+    // batch of two or more right-hand sides per operator
+    builtin_tri_shape(p, p->prec == HH_PREC_MIXED ? 2 : 0, 2, p->real_op && p->real_mode != 0, t);
     if (p->tri_W > 0) {
         t.W = p->tri_W;
         t.mmax = p->tri_mm;
-        t.nr = p->tri_nr;
+        t.nr = p->prec == HH_PREC_MIXED ? 1 : p->tri_nr;
         t.P = (n + t.mmax - 1) / t.mmax;
         if (t.P > n / 2) t.P = n / 2;
     }
@@ -1118,24 +1190,8 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
     t.cstride = p->dim == 2 ? 6L * Ny : 12L * Nz;
     t.lamx = p->lamx; t.lamy = p->lamy;
     int n = p->dim == 2 ? Ny : Nz;
-    long nmodes = p->dim == 2 ? Nx : (long)Nx * Ny;
-    hh::tri_shape(n, nmodes, tri_comp == 0 && tc == 0 ? hh::F64 : hh::F32, &t.W, &t.P, &t.mmax);
-    if (tri_comp == 0 && tc == 1) {   // FP64 elimination on FP32 storage: FP64 register budget
-        hh::tri_shape(n, nmodes, hh::F64, &t.W, &t.P, &t.mmax);
-        fp32_storage_rows(p, n, nmodes, t);
-    }
-    t.nr = 1;
-    // Real bands, FP64 elimination, 16 rows per thread: two right-hand sides
-    // per thread share the operator's pivots (measured at N=1023 x 64: 590
-    // against 654 us per launch in FP64; 419 against 531 us with FP32
-    // storage, where an operator must have two right-hand sides to pair)
-    // when the grid still has 8 blocks per CU or more.
     const long per_op = (batch + p->nop - 1) / p->nop;
-    if (t.real && tri_comp == 0 && t.mmax == 16 && (tc == 0 || per_op >= 2)) {
-        const long tiles = (nmodes + t.W - 1) / t.W;
-        const long groups = ((batch + p->nop - 1) / p->nop + 1) / 2;
-        if (tiles * p->nop * groups >= 2048) t.nr = 2;
-    }
+    builtin_tri_shape(p, tri_comp == 1 ? 2 : tc == 1 ? 1 : 0, per_op, t.real, t);
     if (p->tri_W > 0 && p->tri_mm > 0) {   // shape chosen by the autotuner
         t.W = p->tri_W;
         t.mmax = p->tri_mm;
@@ -1176,6 +1232,9 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
         const int tw = p->tile_w < 0 ? t.W : p->tile_w;
         if (fft && tw > 0) {
             if (!ws || ws == rhs || ws == sol) return fail(HH_ERR_ARG, "internal: missing workspace");
+            // tiles hold whole blocks: the built-in W (a power of two) is cut
+            // to a narrower tile; a shape set by hand must fit as given
+            if (p->tri_W == 0 && t.W > tw) t.W = tw;
             t.tiled = tw;
             hh::Lines fwd = lx, inv = lx;
             fwd.out_tw = tw; fwd.ny = Ny; fwd.field = t.field;

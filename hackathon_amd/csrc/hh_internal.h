@@ -117,6 +117,11 @@ hipError_t launch_compact(const Compact &cd, hipStream_t st);
 
 // Choose a tridiagonal block shape for a system of length n and nmodes modes.
 void tri_shape(int n, long nmodes, int prec, int *W, int *P, int *mmax);
+// LDS bytes of the tridiagonal kernel's reduced system: two buffers of A, C
+// (operator values of op bytes) and nr complex right-hand sides (cx bytes).
+inline size_t tri_lds_bytes(size_t op, size_t cx, int P, int W, int nr) {
+    return 2 * (2 * op + nr * cx) * 2 * (size_t)P * W;
+}
 // block width W and partitions P for mm rows per thread (256-thread blocks at most)
 void tri_shape_rows(int n, long nmodes, int mm, int *W, int *P);
 

@@ -593,12 +593,6 @@ __global__ __launch_bounds__(512) void k_tri_seq(Prob prob, const int *foff, con
         if (s + i < n) prob.store64(b, md, s + i, d[i]);
 }
 
-// LDS bytes of the reduced system: two buffers of A, C (operator values of
-// op bytes) and NR complex right-hand sides (cx bytes).
-inline size_t tri_lds_bytes(size_t op, size_t cx, int P, int W, int nr) {
-    return 2 * (2 * op + nr * cx) * 2 * (size_t)P * W;
-}
-
 template <typename T, int MM, int NR, class Prob>
 static hipError_t launch_tri_nr(const Prob &pb, int n, long nmodes, int batch, int nop, int W, int P,
                                 hipStream_t st) {

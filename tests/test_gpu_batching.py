@@ -93,10 +93,11 @@ def test_fixed_sweeps_3d_mixed(H, ctx, oracle):
 
 
 def test_mixed_3d_255_row_shapes(H, ctx):
-    """3D 255^3 in mixed precision: the built-in shape (16 rows per thread
-    for FP32 storage) and the FP64 one (8 rows) both land within 1e-10 of
-    the FP64 solve of the same right-hand side (the FP64 solver is pinned to
-    the oracle by the parity tests)."""
+    """3D 255^3 in mixed precision: the built-in shape of its FP32
+    elimination (W = 8 modes x 32 chunks of 8 rows), 16 rows per thread and
+    W = 8 x 8 rows all land within 1e-10 of the FP64 solve of the same
+    right-hand side (the FP64 solver is pinned to the oracle by the parity
+    tests)."""
     import ctypes
     import torch
     N = 255
@@ -109,8 +110,8 @@ def test_mixed_3d_255_row_shapes(H, ctx):
     plan.set_refine(0.0, 2)
     v = [ctypes.c_int() for _ in range(4)]
     H._check(H.lib().hh_plan_get_tri_shape(plan.h, *[ctypes.byref(x) for x in v]))
-    assert [x.value for x in v] == [16, 16, 16, 1]
-    for shape in ((0, 0, 0), (8, 8, 1)):
+    assert [x.value for x in v] == [8, 32, 8, 1]
+    for shape in ((0, 0, 0), (16, 16, 1), (8, 8, 1)):
         H._check(H.lib().hh_plan_set_tri_shape(plan.h, *shape))
         sol = plan.solve(rhs).cpu().numpy().ravel()
         err = relerr(sol, ref)
