@@ -1267,7 +1267,9 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
         if ((rc = dst_pass(c, ly, tc, tc, tc, ws, ws))) return rc;
         return dst_pass(c, lx, tc, tc, to, ws, sol);
     }
-    if (tc != to || ti != tc) return fail(HH_ERR_UNSUPPORTED, "mixed precision needs power-of-two N+1 in 3D");
+    // dense 3D path: ping-pong between ws and sol, which holds the
+    // intermediate in the compute type (no larger than the output type)
+    if (tc == 0 && to == 1) return fail(HH_ERR_UNSUPPORTED, "internal: FP64 intermediate in an FP32 output");
     if ((rc = dst_pass(c, lx, ti, tc, tc, rhs, ws))) return rc;
     if ((rc = dst_pass(c, ly, tc, tc, tc, ws, sol))) return rc;
     if ((rc = tri(sol))) return rc;
@@ -1321,9 +1323,10 @@ static int solve_ws(hh_plan *p, const void *rhs, void *sol, int batch, void *scr
     // modes, FP64 substitution of the near-resonant ones, FP64 residual and
     // update; the correction lives in FP32 scratch, followed (2D) by the
     // FP32 mode-tiled spectrum of the solves
+    // (or, with a dense DST, the FP32 intermediate of its out-of-place passes)
     void *dlt = scratch;
-    void *tws = (p->dim == 2 && p->tile_w != 0) ? (void *)((char *)scratch + (size_t)n * 8) : dlt;
-    if (!plan_all_fft(p)) return fail(HH_ERR_UNSUPPORTED, "mixed precision needs power-of-two N+1");
+    const bool own_ws = !plan_all_fft(p) || (p->dim == 2 && p->tile_w != 0);
+    void *tws = own_ws ? (void *)((char *)scratch + (size_t)n * 8) : dlt;
     if (rhs == sol) return fail(HH_ERR_ARG, "mixed precision solve needs d_sol != d_rhs");
     if ((rc = solve_core(p, 0, 1, 0, 1, rhs, sol, tws, batch))) return rc;   // u0 = A32^{-1} F
     if (p->tol == 0) {

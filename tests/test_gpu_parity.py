@@ -289,8 +289,11 @@ def test_rhs3d(H, ctx, oracle, order, kind, p, N):
 
 
 # ------------------------------------------------------------------ mixed precision
-@pytest.mark.parametrize("dim,order,N", [(2, 6, 255), (2, 4, 1023), (2, 6, 2047), (3, 6, 63), (3, 4, 127)])
+@pytest.mark.parametrize("dim,order,N", [(2, 6, 255), (2, 4, 1023), (2, 6, 2047), (3, 6, 63), (3, 4, 127),
+                                         (2, 6, 100), (2, 4, 125), (2, 6, 250), (3, 6, 20), (3, 4, 24)])
 def test_mixed_precision(H, ctx, oracle, dim, order, N):
+    """Every N: power-of-two N+1 on the FFT path, the others (100, 125, 250;
+    3D 20, 24) on the dense DST in FP32."""
     say(f"mixed dim={dim} order={order} N={N}")
     if dim == 2:
         s = oracle.solve2d(order, N, 1, [50.0, 0.0, 3, 7, 0.4], threads=8)
@@ -303,6 +306,34 @@ def test_mixed_precision(H, ctx, oracle, dim, order, N):
     sweeps, rel = plan.last_refine()
     say(f"  sweeps={sweeps} relres={rel:.2e} err={relerr(sol, s['sol']):.2e}")
     assert relerr(sol, s["sol"]) <= 1e-10
+
+
+@pytest.mark.parametrize("dim,N", [(2, 100), (3, 20)])
+def test_mixed_dense_graph(H, ctx, oracle, dim, N):
+    """Mixed precision on the dense DST path in a HIP graph: every replay, with
+    eager solves and synchronisations between them, gives the eager bits."""
+    import torch
+    say(f"mixed dense graph dim={dim} N={N}")
+    if dim == 2:
+        s = oracle.solve2d(6, N, 1, [10.0, 0.0, 3, 2, 0.3], threads=8)
+        plan = H.Plan2D(ctx, N, N, s["coef"], precision=H.PREC_MIXED)
+    else:
+        s = oracle.solve3d(6, N, 1, [10.0, 0.0, 3, 2, 1, 1.0, 1.0], threads=8)
+        plan = H.Plan3D(ctx, N, N, N, s["coef"], precision=H.PREC_MIXED)
+    plan.set_refine(0.0, 3)
+    rhs = dev(s["rhs"])
+    eager = host(plan.solve(rhs))
+    assert relerr(eager, s["sol"]) <= 1e-10
+    out = torch.empty_like(rhs)
+    g = H.Graph(ctx, [(plan, rhs, out)])
+    for rep in range(3):
+        out.zero_()
+        g.launch()
+        torch.cuda.synchronize()
+        assert np.array_equal(host(out), eager), rep
+        plan.solve(rhs)
+        torch.cuda.synchronize()
+    g.close()
 
 
 # ------------------------------------------------------------------ errors
