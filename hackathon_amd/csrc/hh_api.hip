@@ -449,7 +449,7 @@ static int collect_timing(hh_context *c) {
 
 extern "C" int hh_context_set_dst_algorithm(hh_context *c, int algo) {
     if (!c) return fail(HH_ERR_ARG, "null context");
-    if (algo < -1 || algo > 7) return fail(HH_ERR_ARG, "DST algorithm must be -1..7 (got %d)", algo);
+    if (algo < -1 || algo > 15) return fail(HH_ERR_ARG, "DST algorithm must be -1..15 (got %d)", algo);
     std::lock_guard<std::mutex> lk(c->mu);
     c->dst_algo = algo;
     return HH_OK;

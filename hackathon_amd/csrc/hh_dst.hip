@@ -511,6 +511,132 @@ __global__ __launch_bounds__(256) void k_dst_gemm(const C2<Ti> *__restrict__ in,
     }
 }
 
+// Dense DST on the matrix cores: the lines' real and imaginary parts form
+// the rows of a real matrix X (row 2l + c = part c of line l), and
+// OUT = X S is a real GEMM.  Block tile 64 rows (32 lines) x 64 outputs,
+// four waves of 2 x 2 MFMA 16x16 tiles (v_mfma_f64_16x16x4f64 in FP64,
+// v_mfma_f32_16x16x4f32 in FP32, FP32 accumulation), K staged through LDS 16
+// at a time, the next 16 loaded into registers meanwhile; the output tile goes back through LDS so that stores are
+// coalesced for contiguous and for strided lines.  Operand layouts (lane l):
+// A[l % 16][l / 16], B[l / 16][l % 16]; result j of lane l is row
+// l / 16 + 4 j (FP64) or 4 (l / 16) + j (FP32) of column l % 16.
+template <typename T> struct Mfma16;
+template <> struct Mfma16<double> {
+    typedef double V __attribute__((ext_vector_type(4)));
+    static HH_DEV V step(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static HH_DEV int row(int l, int j) { return l / 16 + 4 * j; }
+};
+template <> struct Mfma16<float> {
+    typedef float V __attribute__((ext_vector_type(4)));
+    static HH_DEV V step(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static HH_DEV int row(int l, int j) { return 4 * (l / 16) + j; }
+};
+
+template <typename Ti, typename T, typename To>
+__global__ __launch_bounds__(256) void k_dst_mfma(const C2<Ti> *__restrict__ in, C2<To> *__restrict__ out, Lines ln,
+                                                  long nlines, const T *__restrict__ S) {
+    using M = Mfma16<T>;
+    constexpr int BL = 32, BM = 2 * BL, BN = 64, BK = 16, PAD = 1;
+    // staging [k][m] and [k][n] during the product, the output tile [m][n] after it
+    constexpr int STAGE = BK * (BM + PAD) + BK * (BN + PAD), OUTT = BM * (BN + PAD);
+    __shared__ T sh[STAGE > OUTT ? STAGE : OUTT];
+    T *xs = sh, *ss = sh + BK * (BM + PAD);
+    const int N = ln.N;
+    const long l0 = (long)blockIdx.x * BL;
+    const int n0 = blockIdx.y * BN;
+    const int t = threadIdx.x, lane = t % 64, wv = t / 64;
+    const int wm = (wv % 2) * 32, wn = (wv / 2) * 32;   // the wave's 32 x 32 sub-tile
+    auto line_base = [&](long l) -> long {
+        const long p = l / ln.ncols, c = l % ln.ncols;
+        return p * ln.plane_stride + c * ln.col_stride;
+    };
+    typename M::V acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = typename M::V{0, 0, 0, 0};
+    // chunk k0 of X (BL lines x BK elements; contiguous lines: threads along
+    // the line, strided lines: threads across lines) and of S, into
+    // registers: the next chunk's loads are in flight while the matrix cores
+    // work on the current one
+    constexpr int XI = BL * BK / 256, SI = BK * BN / 256;
+    C2<T> xv[XI];
+    T sv[SI];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < XI; i++) {
+            const int e = t + 256 * i;
+            int a, b;
+            if (ln.elem_stride == 1) { a = e / BK; b = e % BK; }
+            else { a = e % BL; b = e / BL; }
+            const long l = l0 + a;
+            const int k = k0 + b;
+            xv[i] = (l < nlines && k < N) ? cconv<T, Ti>(in[line_base(l) + (long)k * ln.elem_stride]) : mk<T>(0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < SI; i++) {
+            const int e = t + 256 * i;
+            const int b = e / BN, c = e % BN;
+            const int k = k0 + b, n = n0 + c;
+            sv[i] = (k < N && n < N) ? S[(long)k * N + n] : (T)0;
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < N; k0 += BK) {
+#pragma unroll
+        for (int i = 0; i < XI; i++) {
+            const int e = t + 256 * i;
+            int a, b;
+            if (ln.elem_stride == 1) { a = e / BK; b = e % BK; }
+            else { a = e % BL; b = e / BL; }
+            xs[b * (BM + PAD) + 2 * a] = xv[i].x;
+            xs[b * (BM + PAD) + 2 * a + 1] = xv[i].y;
+        }
+#pragma unroll
+        for (int i = 0; i < SI; i++) {
+            const int e = t + 256 * i;
+            ss[(e / BN) * (BN + PAD) + e % BN] = sv[i];
+        }
+        __syncthreads();
+        if (k0 + BK < N) load(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int kr = kk + lane / 16;
+            T a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) a[i] = xs[kr * (BM + PAD) + wm + 16 * i + lane % 16];
+#pragma unroll
+            for (int j = 0; j < 2; j++) b[j] = ss[kr * (BN + PAD) + wn + 16 * j + lane % 16];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) acc[i][j] = M::step(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+    // output tile through LDS
+    T *os = sh;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                os[(wm + 16 * i + M::row(lane, q)) * (BN + PAD) + wn + 16 * j + lane % 16] = acc[i][j][q];
+    __syncthreads();
+#pragma unroll
+    for (int e = t; e < BL * BN; e += 256) {
+        int a, c;
+        if (ln.elem_stride == 1) { a = e / BN; c = e % BN; }
+        else { a = e % BL; c = e / BL; }
+        const long l = l0 + a;
+        const int n = n0 + c;
+        if (l < nlines && n < N)
+            out[line_base(l) + (long)n * ln.elem_stride] =
+                cconv<To, T>(mk<T>(os[2 * a * (BN + PAD) + c], os[(2 * a + 1) * (BN + PAD) + c]));
+    }
+}
+
 bool fft_supported(int N) {
     int L = 2 * (N + 1);
     return N >= 1 && L <= HH_FFT_LMAX && (L & (L - 1)) == 0 && L >= 4;
@@ -655,8 +781,12 @@ static hipError_t launch_dst_t(const Lines &ln, int algo, const void *in, void *
     if (in == out || S == nullptr) return hipErrorInvalidValue;
     long nlines = ln.ncols * ln.nplanes;
     dim3 grid((unsigned)((nlines + 31) / 32), (unsigned)((ln.N + 63) / 64));
-    hipLaunchKernelGGL((k_dst_gemm<Ti, T, To>), grid, dim3(256), 0, st, (const C2<Ti> *)in, (C2<To> *)out, ln,
-                       nlines, (const T *)S);
+    if (algo & 8)   // vector-FMA form (kept for comparison)
+        hipLaunchKernelGGL((k_dst_gemm<Ti, T, To>), grid, dim3(256), 0, st, (const C2<Ti> *)in, (C2<To> *)out, ln,
+                           nlines, (const T *)S);
+    else
+        hipLaunchKernelGGL((k_dst_mfma<Ti, T, To>), grid, dim3(256), 0, st, (const C2<Ti> *)in, (C2<To> *)out, ln,
+                           nlines, (const T *)S);
     return hipGetLastError();
 }
 

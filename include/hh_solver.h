@@ -68,7 +68,9 @@ int hh_context_enable_timing(hh_context *ctx, int on);
  * its inverse at N+1 = 4096, against 5e-16).  -1 (default): the
  * (N+1)-point transform, in 64-thread blocks for contiguous FP64 rows with
  * N+1 >= 512, 512-thread blocks for strided lines in FP32 at 128 <= N+1 <=
- * 256, and 256-thread blocks otherwise. */
+ * 256, and 256-thread blocks otherwise.  Other N+1 take the dense DST (the
+ * orthonormal sine matrix times the lines) on the matrix cores (FP64 or
+ * FP32 MFMA); bit 3 of algo selects the vector-FMA form of it instead. */
 int hh_context_set_dst_algorithm(hh_context *ctx, int algo);
 int hh_context_read_timing(hh_context *ctx, double ms[5], long launches[5], double bytes[5]);
 int hh_context_reset_timing(hh_context *ctx);

@@ -46,11 +46,12 @@ def relerr(a, b):
 
 
 # ------------------------------------------------------------------ DST
-@pytest.fixture(params=[-1, 0, 1, 2, 3, 4], ids=["auto", "half", "oddext", "half64", "oddext64", "halfwidecols"])
+@pytest.fixture(params=[-1, 0, 1, 2, 3, 4, 8],
+                ids=["auto", "half", "oddext", "half64", "oddext64", "halfwidecols", "densefma"])
 def dctx(ctx, request):
     """The context with each DST variant: (N+1)-point or odd-extension FFT, 256- or 64-thread blocks,
     512-thread blocks for strided lines, and the automatic choice ((N+1)-point, block size by line
-    length)."""
+    length); for other N+1 the dense DST on the matrix cores, or (bit 3) by vector FMA."""
     ctx.set_dst_algorithm(request.param)
     yield ctx
     ctx.set_dst_algorithm(-1)
@@ -210,6 +211,7 @@ CASES3D = [
     (2, 0, [10, 9, 10, 9], 15), (4, 0, [10, 9, 10, 9], 31), (6, 0, [10, 9, 10, 9], 31),
     (6, 0, [10, 9, 10, 9], 20), (6, 1, [30.0, 0.0, 3, 2, 1, 1.0, 1.0], 63),
     (4, 1, [30.0, 0.0, 5, 1, 2, 1.0, 1.0], 63), (6, 1, [30.0, 0.0, 3, 2, 1, 1.0, 1.0], 24),
+    (6, 1, [30.0, 0.0, 3, 2, 1, 1.0, 1.0], 125),   # the paper's 125^3: dense DST on the matrix cores
 ]
 
 
