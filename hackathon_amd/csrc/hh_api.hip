@@ -305,7 +305,10 @@ static int dst_algo_lines(const hh_context *c, const hh::Lines &ln, int tc) {
 }
 
 // One DST pass over lines; in-place when the FFT path applies.
-static int dst_pass(hh_context *c, const hh::Lines &ln, int ti, int tc, int to, const void *in, void *out) {
+static int dst_pass(hh_context *c, const hh::Lines &ln0, int ti, int tc, int to, const void *in, void *out,
+                    double acc = 0) {
+    hh::Lines ln = ln0;
+    ln.acc = acc;
     void *tw = nullptr, *S = nullptr;
     int rc;
     const int algo = dst_algo_lines(c, ln, tc);
@@ -314,7 +317,7 @@ static int dst_pass(hh_context *c, const hh::Lines &ln, int ti, int tc, int to, 
     else rc = get_dst_matrix(c, ln.N, tc, &S);
     if (rc) return rc;
     const double elems = (double)ln.N * ln.ncols * ln.nplanes;
-    TimeScope ts(c, tw ? 0 : 1, elems * ((ti ? 8 : 16) + (to ? 8 : 16)));
+    TimeScope ts(c, tw ? 0 : 1, elems * ((ti ? 8 : 16) + (to ? 8 : 16) * (acc != 0 ? 2 : 1)));
     hh::DstTypes ty{ti, tc, to};
     ty.algo = algo;
     hipError_t e = hh::launch_dst(ln, ty, in, out, tw, S, c->stream);
@@ -1174,8 +1177,10 @@ extern "C" int hh_plan_last_refine(const hh_plan *p, int *sweeps, double *rel) {
 // Transform-solve-transform on one buffer.  Types: ti input, tc compute /
 // intermediate storage, to output (0 = FP64, 1 = FP32).  tri_comp is the
 // arithmetic of the tridiagonal elimination.
+// acc != 0: sol += acc A^{-1} rhs, the last pass accumulating into the FP64
+// sol (rhs is then consumed: it may serve as scratch).
 static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const void *rhs, void *sol, void *ws,
-                      int batch) {
+                      int batch, double acc = 0) {
     hh_context *c = p->ctx;
     const int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz;
     int rc;
@@ -1241,18 +1246,19 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
             inv.in_tw = tw; inv.ny = Ny; inv.field = t.field;
             if ((rc = dst_pass(c, fwd, ti, tc, tc, rhs, ws))) return rc;
             if ((rc = tri(ws))) return rc;
-            return dst_pass(c, inv, tc, tc, to, ws, sol);
+            return dst_pass(c, inv, tc, tc, to, ws, sol, acc);
         }
         // row-major spectrum: in sol (in place) on the FFT path, in ws otherwise
-        void *mid = fft ? (tc == to ? sol : ws) : ws;
+        void *mid = fft ? (tc == to && acc == 0 ? sol : ws) : ws;
         if (!mid) return fail(HH_ERR_ARG, "internal: missing workspace");
         if ((rc = dst_pass(c, lx, ti, tc, tc, rhs, mid))) return rc;
         if ((rc = tri(mid))) return rc;
-        return dst_pass(c, lx, tc, tc, to, mid, sol);
+        return dst_pass(c, lx, tc, tc, to, mid, sol, acc);
     }
+    if (acc != 0 && (to != 0 || ti != tc)) return fail(HH_ERR_UNSUPPORTED, "internal: accumulating solve types");
     hh::Lines lx = rows_lines(Nx, (long)Ny * Nz * batch);
     hh::Lines ly = cols_lines(Nx, Ny, (long)Nz * batch);
-    if (fft && tc == to) {
+    if (fft && tc == to && acc == 0) {
         if ((rc = dst_pass(c, lx, ti, tc, tc, rhs, sol))) return rc;
         if ((rc = dst_pass(c, ly, tc, tc, tc, sol, sol))) return rc;
         if ((rc = tri(sol))) return rc;
@@ -1265,16 +1271,19 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
         if ((rc = dst_pass(c, ly, tc, tc, tc, ws, ws))) return rc;
         if ((rc = tri(ws))) return rc;
         if ((rc = dst_pass(c, ly, tc, tc, tc, ws, ws))) return rc;
-        return dst_pass(c, lx, tc, tc, to, ws, sol);
+        return dst_pass(c, lx, tc, tc, to, ws, sol, acc);
     }
-    // dense 3D path: ping-pong between ws and sol, which holds the
-    // intermediate in the compute type (no larger than the output type)
+    // dense 3D path: ping-pong between ws and a second buffer holding the
+    // intermediate in the compute type: sol (no smaller than the compute
+    // type), or, when sol is accumulated into, the consumed rhs
     if (tc == 0 && to == 1) return fail(HH_ERR_UNSUPPORTED, "internal: FP64 intermediate in an FP32 output");
+    void *mid = acc != 0 ? const_cast<void *>(rhs) : sol;
+    if (mid == ws) return fail(HH_ERR_ARG, "internal: workspace aliases the intermediate");
     if ((rc = dst_pass(c, lx, ti, tc, tc, rhs, ws))) return rc;
-    if ((rc = dst_pass(c, ly, tc, tc, tc, ws, sol))) return rc;
-    if ((rc = tri(sol))) return rc;
-    if ((rc = dst_pass(c, ly, tc, tc, tc, sol, ws))) return rc;
-    return dst_pass(c, lx, tc, tc, to, ws, sol);
+    if ((rc = dst_pass(c, ly, tc, tc, tc, ws, mid))) return rc;
+    if ((rc = tri(mid))) return rc;
+    if ((rc = dst_pass(c, ly, tc, tc, tc, mid, ws))) return rc;
+    return dst_pass(c, lx, tc, tc, to, ws, sol, acc);
 }
 
 static int residual_into(hh_plan *p, const void *u, const void *F, void *res, int res_f32, int batch) {
@@ -1332,12 +1341,11 @@ static int solve_ws(hh_plan *p, const void *rhs, void *sol, int batch, void *scr
     if (p->tol == 0) {
         // fixed number of sweeps: no norms, no host synchronisation, so the
         // whole solve can be recorded into a HIP graph
+        // (each sweep: r = A u - F, then u -= A32^{-1} r with the update
+        // applied by the last DST pass as it stores)
         for (int it = 0; it < p->max_iter; it++) {
             if ((rc = residual_into(p, sol, rhs, dlt, 1, batch))) return rc;
-            if ((rc = solve_core(p, 1, 1, 1, 1, dlt, dlt, tws, batch))) return rc;
-            TimeScope ts(c, 3, (double)n * 40);
-            hipError_t e = hh::launch_axpy_update(n, sol, dlt, 1, -1.0, c->stream);
-            if (e != hipSuccess) return fail(HH_ERR_DEVICE, "update launch failed: %s", hipGetErrorString(e));
+            if ((rc = solve_core(p, 1, 1, 0, 1, dlt, sol, tws, batch, -1.0))) return rc;
         }
         p->last_sweeps = p->max_iter;
         p->last_rel = -1.0;
@@ -1355,10 +1363,7 @@ static int solve_ws(hh_plan *p, const void *rhs, void *sol, int batch, void *scr
         if ((rc = norms_sync(c, n, dlt, 1, nullptr, nr))) return rc;
         rel = nr[0] / fn;
         if (rel <= p->tol || sweeps >= p->max_iter) break;
-        if ((rc = solve_core(p, 1, 1, 1, 1, dlt, dlt, tws, batch))) return rc;   // A d = r
-        TimeScope ts(c, 3, (double)n * 40);
-        hipError_t e = hh::launch_axpy_update(n, sol, dlt, 1, -1.0, c->stream);    // u -= d
-        if (e != hipSuccess) return fail(HH_ERR_DEVICE, "update launch failed: %s", hipGetErrorString(e));
+        if ((rc = solve_core(p, 1, 1, 0, 1, dlt, sol, tws, batch, -1.0))) return rc;   // u -= A32^{-1} r
         sweeps++;
     }
     p->last_sweeps = sweeps;

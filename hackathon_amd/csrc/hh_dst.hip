@@ -71,6 +71,7 @@ struct LinePos {
 struct RowMap {
     int N, Ny, tw;
     long field;
+    double acc;   // output map: nonzero = accumulate (see Lines::acc)
     HH_DEV LinePos line(long row) const {
         LinePos p;
         p.N = N; p.Ny = Ny; p.tw = tw;
@@ -89,6 +90,19 @@ struct RowMap {
         return p;
     }
 };
+
+// Store of one output element: out = X, or out += acc X for an FP64 output
+// with acc != 0 (the refinement update folded into the last pass).
+template <typename To, typename T> HH_DEV void put_out(C2<To> *o, C2<T> v, double acc) {
+    if constexpr (sizeof(To) == 8) {
+        if (acc != 0) {
+            const C2<double> u = *o;
+            *o = mk<double>(u.x + acc * (double)v.x, u.y + acc * (double)v.y);
+            return;
+        }
+    }
+    *o = cconv<To, T>(v);
+}
 
 // Coalesced load of LPB consecutive lines from the mode-tiled layout into
 // the LDS line buffers (element k of line l at position k+1), in tile-major
@@ -159,7 +173,7 @@ __global__ __launch_bounds__((dst_cfg<L, TPB>::THREADS)) void k_dst_fft_rows(
         return zero ? mk<T>(0, 0) : r;
     };
     auto put = [&](int k, C2<T> z) {
-        if (valid && k >= 1 && k <= N) out[lout.at(k - 1)] = cconv<To, T>(mk<T>(-z.y * scale, z.x * scale));
+        if (valid && k >= 1 && k <= N) put_out<To, T>(out + lout.at(k - 1), mk<T>(-z.y * scale, z.x * scale), mout.acc);
     };
     fft_line<T, L, false>(sre, sim, tw, t, get, put);
 }
@@ -390,7 +404,7 @@ __global__ __launch_bounds__((half_cfg<M, TPB>::THREADS)) void k_dst_half_rows(c
             const int e = threadIdx.x + i * NT;
             if (e < nr * N) {
                 const int l = e / N, m = e % N + 1;
-                y[e] = cconv<To, T>(mk<T>(smem[l * LS + lds_pad(m)], smem[l * LS + LP + lds_pad(m)]));
+                put_out<To, T>(y + e, mk<T>(smem[l * LS + lds_pad(m)], smem[l * LS + LP + lds_pad(m)]), mout.acc);
             }
         }
     } else {
@@ -408,8 +422,8 @@ __global__ __launch_bounds__((half_cfg<M, TPB>::THREADS)) void k_dst_half_rows(c
             const int rem = e & ((1 << (lg + LGL)) - 1);
             const int l = rem >> lg, k = (tile << lg) + (rem & (tw - 1));
             if (k < N && l < nr)
-                out[mout.next(l0, l).at(k)] =
-                    cconv<To, T>(mk<T>(smem[l * LS + lds_pad(k + 1)], smem[l * LS + LP + lds_pad(k + 1)]));
+                put_out<To, T>(out + mout.next(l0, l).at(k),
+                               mk<T>(smem[l * LS + lds_pad(k + 1)], smem[l * LS + LP + lds_pad(k + 1)]), mout.acc);
         }
     }
 }
@@ -506,7 +520,7 @@ __global__ __launch_bounds__(256) void k_dst_gemm(const C2<Ti> *__restrict__ in,
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             int k = k0 + tk + 8 * u;
-            if (k < N) out[bse + (long)k * ln.elem_stride] = cconv<To, T>(acc[u]);
+            if (k < N) put_out<To, T>(out + bse + (long)k * ln.elem_stride, acc[u], ln.acc);
         }
     }
 }
@@ -632,8 +646,8 @@ __global__ __launch_bounds__(256) void k_dst_mfma(const C2<Ti> *__restrict__ in,
         const long l = l0 + a;
         const int n = n0 + c;
         if (l < nlines && n < N)
-            out[line_base(l) + (long)n * ln.elem_stride] =
-                cconv<To, T>(mk<T>(os[2 * a * (BN + PAD) + c], os[(2 * a + 1) * (BN + PAD) + c]));
+            put_out<To, T>(out + line_base(l) + (long)n * ln.elem_stride,
+                           mk<T>(os[2 * a * (BN + PAD) + c], os[(2 * a + 1) * (BN + PAD) + c]), ln.acc);
     }
 }
 
@@ -659,12 +673,12 @@ static hipError_t launch_fft_L(const Lines &ln, const void *in, void *out, const
             if (e != hipSuccess) return e;
         }
         long grid = (nrows + cfg::LPB - 1) / cfg::LPB;
-        const RowMap rin{ln.N, ln.ny, ln.in_tw, ln.field}, rout{ln.N, ln.ny, ln.out_tw, ln.field};
+        const RowMap rin{ln.N, ln.ny, ln.in_tw, ln.field, 0.0}, rout{ln.N, ln.ny, ln.out_tw, ln.field, ln.acc};
         hipLaunchKernelGGL(kfn, dim3((unsigned)grid), dim3(cfg::THREADS), shm, st, pi, po, nrows, scale, ptw, rin,
                            rout);
         return hipGetLastError();
     }
-    if (ln.col_stride != 1) return hipErrorInvalidValue;
+    if (ln.col_stride != 1 || ln.acc != 0) return hipErrorInvalidValue;   // strided lines: plain stores
     auto kfn = k_dst_fft_cols<Ti, T, To, L, TPB>;
     if (shm > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
@@ -695,12 +709,12 @@ static hipError_t launch_half_M(const Lines &ln, const void *in, void *out, cons
             if (e != hipSuccess) return e;
         }
         long grid = (nrows + cfg::LPB - 1) / cfg::LPB;
-        const RowMap rin{ln.N, ln.ny, ln.in_tw, ln.field}, rout{ln.N, ln.ny, ln.out_tw, ln.field};
+        const RowMap rin{ln.N, ln.ny, ln.in_tw, ln.field, 0.0}, rout{ln.N, ln.ny, ln.out_tw, ln.field, ln.acc};
         hipLaunchKernelGGL(kfn, dim3((unsigned)grid), dim3(cfg::THREADS), shm, st, pi, po, nrows, scale, pa, rin,
                            rout);
         return hipGetLastError();
     }
-    if (ln.col_stride != 1) return hipErrorInvalidValue;
+    if (ln.col_stride != 1 || ln.acc != 0) return hipErrorInvalidValue;   // strided lines: plain stores
     if constexpr (TPB == 256 && M >= 128 && M <= 256) {
         using wcfg = half_cfg<M, 512>;
         constexpr int WNW = wcfg::P > 64 ? wcfg::P / 64 : 1;

@@ -36,6 +36,10 @@ struct Lines {
     // (j, k) at t0*ny + j*w + (k - t0), w = min(tw, N - t0).
     int in_tw = 0, out_tw = 0, ny = 1;
     long field = 0;
+    // Contiguous lines only: acc != 0 adds acc times the transform to the
+    // (FP64) output instead of storing it, out += acc X (the update of an
+    // iterative refinement applied as the last pass stores).
+    double acc = 0;
 };
 
 bool fft_supported(int N);  // N+1 a power of two, 2(N+1) <= HH_FFT_LMAX
