@@ -54,7 +54,8 @@ def build(verbose=False):
     with cf.ThreadPoolExecutor(jobs) as ex:
         objs = list(ex.map(_compile, CORE_SOURCES))
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-L/opt/rocm/lib",
+          "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     # drop-in reference-shaped front ends (plain C, link against libhh.so)
     for src, out in (("hh_ref2d.c", REF2D), ("hh_ref3d.c", REF3D)):

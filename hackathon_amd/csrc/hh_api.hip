@@ -5,6 +5,7 @@
 // own the device copies of the O(N) stencil coefficient vectors and the
 // eigenvalues 2cos(k pi/(N+1)); they replace the reference's LU() factors.
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cmath>
 #include <complex>
@@ -79,11 +80,24 @@ struct hh_context {
 };
 
 // Records a pair of events around the launches in its scope when timing is on.
+// Phase ranges for rocprofv3 --marker-trace (the reference's NVTX ranges,
+// 2D_Program/solver_functions/cuda_helper.h:13-25 around the phases of
+// solver.c:16-92): one roctx range per kernel pass, named by category, and
+// one per solve; they cost a call each when no profiler listens.
+static const char *const kPhaseName[] = {"hh:dst", "hh:dst_dense", "hh:tridiagonal", "hh:residual_update",
+                                         "hh:rhs"};
+struct Range {
+    explicit Range(const char *name) { roctxRangePushA(name); }
+    ~Range() { roctxRangePop(); }
+};
+
 struct TimeScope {
     hh_context *c;
     hh_context::Rec r;
     bool on;
-    TimeScope(hh_context *ctx, int cat, double bytes) : c(ctx), on(ctx->timing && !ctx->no_alloc) {
+    Range range;
+    TimeScope(hh_context *ctx, int cat, double bytes)
+        : c(ctx), on(ctx->timing && !ctx->no_alloc), range(kPhaseName[cat]) {
         if (!on) return;
         r.cat = cat;
         r.bytes = bytes;
@@ -1297,8 +1311,8 @@ static int residual_into(hh_plan *p, const void *u, const void *F, void *res, in
     return HH_OK;
 }
 
-static int norms_sync(hh_context *c, long n, const void *x, int x_f32, const void *y, double out[2]) {
-    hipError_t e = hh::launch_norms(n, x, x_f32, y, c->d_norm, c->stream);
+static int norms_sync(hh_context *c, long n, const void *x, int x_f32, const void *y, double out[2], bool l2 = false) {
+    hipError_t e = hh::launch_norms(n, x, x_f32, y, c->d_norm, c->stream, l2);
     if (e != hipSuccess) return fail(HH_ERR_DEVICE, "norm launch failed: %s", hipGetErrorString(e));
     HIPCHK(hipMemcpyAsync(out, c->d_norm, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1398,6 +1412,7 @@ extern "C" int hh_solve_noalloc(hh_plan *p, const void *rhs, void *sol, int batc
                                         "its static workspace (use hh_solve, or a smaller batch)",
                     ws, HH_NOALLOC_BYTES);
     c->no_alloc = true;
+    Range range("hh_solve_noalloc");
     rc = solve_ws(p, rhs, sol, batch, c->noalloc_ws);
     c->no_alloc = false;
     return rc;
@@ -1407,6 +1422,7 @@ extern "C" int hh_solve(hh_plan *p, const void *rhs, void *sol, int batch) {
     if (!p || !rhs || !sol) return fail(HH_ERR_ARG, "hh_solve: null argument");
     if (batch < 1) return fail(HH_ERR_ARG, "hh_solve: batch must be >= 1 (got %d)", batch);
     std::lock_guard<std::mutex> lk(p->ctx->mu);
+    Range range("hh_solve");
     return solve_locked(p, rhs, sol, batch);
 }
 
@@ -1523,6 +1539,7 @@ extern "C" int hh_graph_launch(hh_graph *G) {
         if (pg.first->gen != pg.second)
             return fail(HH_ERR_ARG, "hh_graph_launch: a plan of the graph changed since it was recorded "
                                     "(tables reallocated, band kind, operator count or shapes); record it again");
+    Range range("hh_graph_launch");
     HIPCHK(hipGraphLaunch(G->exec, c->stream));
     return HH_OK;
 }
@@ -1734,9 +1751,19 @@ extern "C" int hh_rhs3d(hh_plan *p, int order, double hx, double hy, double hz, 
 }
 
 extern "C" int hh_norms(hh_context *c, long n, const void *x, const void *y, double out[2]) {
-    if (!c || !x || !out || n < 0) return fail(HH_ERR_ARG, "hh_norms: bad arguments");
+    if (!c || !out || n < 0 || (n > 0 && !x)) return fail(HH_ERR_ARG, "hh_norms: bad arguments");
+    if (n == 0) { out[0] = out[1] = 0.0; return HH_OK; }   // norms of an empty vector (x may be null)
     std::lock_guard<std::mutex> lk(c->mu);
     int rc = set_device(c);
     if (rc) return rc;
     return norms_sync(c, n, x, 0, y, out);
+}
+
+extern "C" int hh_norms_l2(hh_context *c, long n, const void *x, const void *y, double out[2]) {
+    if (!c || !out || n < 0 || (n > 0 && !x)) return fail(HH_ERR_ARG, "hh_norms_l2: bad arguments");
+    if (n == 0) { out[0] = out[1] = 0.0; return HH_OK; }   // norms of an empty vector (x may be null)
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = set_device(c);
+    if (rc) return rc;
+    return norms_sync(c, n, x, 0, y, out, true);
 }

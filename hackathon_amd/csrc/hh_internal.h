@@ -172,7 +172,9 @@ hipError_t launch_axpy_update(long n, void *u, const void *c, int c_f32, double 
 // memory of norm_buffer_doubles() values, out[2..] the per-block partials
 constexpr int NORM_BLOCKS = 2048;
 constexpr long norm_buffer_doubles() { return 2 + 2L * NORM_BLOCKS; }
-hipError_t launch_norms(long n, const void *x, int x_f32, const void *y, double *out, hipStream_t st);
+// out[0..1] = the max norms (l2: the 2-norms) of x and x - y; out[2..] scratch
+hipError_t launch_norms(long n, const void *x, int x_f32, const void *y, double *out, hipStream_t st,
+                        bool l2 = false);
 hipError_t launch_convert(long n, const void *in, int in_f32, void *out, int out_f32, hipStream_t st);
 
 }  // namespace hh

@@ -10,6 +10,8 @@
 // pointwise pass, which also subtracts the Dirichlet closure.
 // Residual: 2D_Program/main_functions/residual.c:3-34 (9-point),
 //           3D_Program/main_functions/residual.c:42-96 (27-point).
+#include <type_traits>
+
 #include "hh_common.h"
 #include "hh_internal.h"
 
@@ -484,13 +486,18 @@ hipError_t launch_convert(long n, const void *in, int in_f32, void *out, int out
     return hipGetLastError();
 }
 
-// max |x| and max |x - y| over n complex values, in two passes without
-// atomics: each block writes the maxima of |x|^2 and |x - y|^2 over its
-// share to part[blockIdx.x] and part[NORM_BLOCKS + blockIdx.x], then one
-// block combines them and roots the result once (sqrt is correctly rounded
-// and monotonic, so this equals the maximum of the rooted values).  Four
-// values per thread and step are loaded before any is used.
-template <typename Tx>
+// max |x| and max |x - y| (L2 = false), or sqrt(sum |x|^2) and
+// sqrt(sum |x - y|^2) (L2 = true), over n complex values, in two passes
+// without atomics: each block writes the maxima (sums) of |x|^2 and
+// |x - y|^2 over its share to part[blockIdx.x] and part[NORM_BLOCKS +
+// blockIdx.x], then one block combines them and roots the result once (sqrt
+// is correctly rounded and monotonic, so the maximum equals the maximum of
+// the rooted values).  The grid and the reduction tree are fixed, so a sum
+// is the same bits from run to run.  Four values per thread and step are
+// loaded before any is used.
+template <bool L2> HH_DEV double norm_join(double a, double b) { return L2 ? a + b : fmax(a, b); }
+
+template <typename Tx, bool L2>
 __global__ __launch_bounds__(256) void k_norms(long n, const C2<Tx> *x, const z2 *y, double *part) {
     __shared__ double s0[256], s1[256];
     double m0 = 0, m1 = 0;
@@ -506,22 +513,22 @@ __global__ __launch_bounds__(256) void k_norms(long n, const C2<Tx> *x, const z2
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-            m0 = fmax(m0, cabs2<double>(a[u]));
-            if (y) m1 = fmax(m1, cabs2<double>(zd(a[u], c[u])));
+            m0 = norm_join<L2>(m0, cabs2<double>(a[u]));
+            if (y) m1 = norm_join<L2>(m1, cabs2<double>(zd(a[u], c[u])));
         }
     }
     for (; id < n; id += stride) {
         const z2 a = cconv<double, Tx>(x[id]);
-        m0 = fmax(m0, cabs2<double>(a));
-        if (y) m1 = fmax(m1, cabs2<double>(zd(a, y[id])));
+        m0 = norm_join<L2>(m0, cabs2<double>(a));
+        if (y) m1 = norm_join<L2>(m1, cabs2<double>(zd(a, y[id])));
     }
     s0[threadIdx.x] = m0;
     s1[threadIdx.x] = m1;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
-            s0[threadIdx.x] = fmax(s0[threadIdx.x], s0[threadIdx.x + s]);
-            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + s]);
+            s0[threadIdx.x] = norm_join<L2>(s0[threadIdx.x], s0[threadIdx.x + s]);
+            s1[threadIdx.x] = norm_join<L2>(s1[threadIdx.x], s1[threadIdx.x + s]);
         }
         __syncthreads();
     }
@@ -531,20 +538,21 @@ __global__ __launch_bounds__(256) void k_norms(long n, const C2<Tx> *x, const z2
     }
 }
 
+template <bool L2>
 __global__ __launch_bounds__(256) void k_norms_final(int nb, const double *part, double *out) {
     __shared__ double s0[256], s1[256];
     double m0 = 0, m1 = 0;
     for (int b = threadIdx.x; b < nb; b += 256) {
-        m0 = fmax(m0, part[b]);
-        m1 = fmax(m1, part[NORM_BLOCKS + b]);
+        m0 = norm_join<L2>(m0, part[b]);
+        m1 = norm_join<L2>(m1, part[NORM_BLOCKS + b]);
     }
     s0[threadIdx.x] = m0;
     s1[threadIdx.x] = m1;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) {
-            s0[threadIdx.x] = fmax(s0[threadIdx.x], s0[threadIdx.x + s]);
-            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + s]);
+            s0[threadIdx.x] = norm_join<L2>(s0[threadIdx.x], s0[threadIdx.x + s]);
+            s1[threadIdx.x] = norm_join<L2>(s1[threadIdx.x], s1[threadIdx.x + s]);
         }
         __syncthreads();
     }
@@ -554,19 +562,23 @@ __global__ __launch_bounds__(256) void k_norms_final(int nb, const double *part,
     }
 }
 
-hipError_t launch_norms(long n, const void *x, int x_f32, const void *y, double *out, hipStream_t st) {
+hipError_t launch_norms(long n, const void *x, int x_f32, const void *y, double *out, hipStream_t st, bool l2) {
     long blocks = (n + 2047) / 2048;   // about eight values per thread
     if (blocks > NORM_BLOCKS) blocks = NORM_BLOCKS;
     if (blocks < 1) blocks = 1;
     double *part = out + 2;
-    if (x_f32)
-        hipLaunchKernelGGL(k_norms<float>, dim3((unsigned)blocks), dim3(256), 0, st, n, (const float2 *)x,
-                           (const z2 *)y, part);
-    else
-        hipLaunchKernelGGL(k_norms<double>, dim3((unsigned)blocks), dim3(256), 0, st, n, (const z2 *)x,
-                           (const z2 *)y, part);
-    hipLaunchKernelGGL(k_norms_final, dim3(1), dim3(256), 0, st, (int)blocks, (const double *)part, out);
-    return hipGetLastError();
+    auto run = [&](auto L) -> hipError_t {
+        constexpr bool B = decltype(L)::value;
+        if (x_f32)
+            hipLaunchKernelGGL((k_norms<float, B>), dim3((unsigned)blocks), dim3(256), 0, st, n, (const float2 *)x,
+                               (const z2 *)y, part);
+        else
+            hipLaunchKernelGGL((k_norms<double, B>), dim3((unsigned)blocks), dim3(256), 0, st, n, (const z2 *)x,
+                               (const z2 *)y, part);
+        hipLaunchKernelGGL(k_norms_final<B>, dim3(1), dim3(256), 0, st, (int)blocks, (const double *)part, out);
+        return hipGetLastError();
+    };
+    return l2 ? run(std::true_type()) : run(std::false_type());
 }
 
 }  // namespace hh

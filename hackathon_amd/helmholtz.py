@@ -170,6 +170,15 @@ class Context:
                               _dptr(y) if y is not None else ctypes.c_void_p(0), out))
         return out[0], out[1]
 
+    def norms_l2(self, x, y=None):
+        """(sqrt(sum |x|^2), sqrt(sum |x - y|^2)) over all entries, on the device."""
+        _require_cuda(x, "x")
+        out = (ctypes.c_double * 2)()
+        self.bind_stream()
+        _check(lib().hh_norms_l2(self.h, ctypes.c_long(x.numel()), _dptr(x),
+                                 _dptr(y) if y is not None else ctypes.c_void_p(0), out))
+        return out[0], out[1]
+
 
 # ------------------------------------------------------------------ coefficients
 def coefficients2d(order, Nx, Ny, hx, hy, k2e, k2y=None, k2yy=None):

@@ -312,6 +312,10 @@ int hh_rhs3d_host(hh_plan *plan, int order, double hx, double hy, double hz, con
 /* ---- norms: normInf of 2D_Program/helpers_functions/norms.c:3-13 ----
  * out[0] = max |x|, out[1] = max |x - y| (0 when d_y is NULL); blocks. */
 int hh_norms(hh_context *ctx, long n, const void *d_x, const void *d_y, double out[2]);
+/* normL2 of 3D_Program/helpers_functions/norms.c:15-24 on the device:
+ * out[0] = sqrt(sum |x|^2), out[1] = sqrt(sum |x - y|^2) (0 when d_y is
+ * NULL), summed in a fixed order (the same bits from run to run); blocks. */
+int hh_norms_l2(hh_context *ctx, long n, const void *d_x, const void *d_y, double out[2]);
 
 /* ---- serving: continuous batching of mixed 2D/3D requests ----
  * No reference counterpart (the reference's main() solves one problem per

@@ -338,6 +338,28 @@ def test_mixed_dense_graph(H, ctx, oracle, dim, N):
     g.close()
 
 
+# ------------------------------------------------------------------ norms
+@pytest.mark.parametrize("n", [0, 1, 1000, 1_000_003])
+def test_norms_vs_numpy(H, ctx, n):
+    """normInf (2D norms.c:3-13) and normL2 (3D norms.c:15-24) on the device
+    against their definitions: max |x| to the last bits of a square root,
+    sqrt(sum |x|^2) to 1e-13 (summation order); the sum is the same bits from
+    run to run."""
+    say(f"norms n={n}")
+    rng = np.random.default_rng(n + 7)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    y = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    dx, dy = dev(x), dev(y)
+    mi, md = ctx.norms(dx, dy)
+    l2, l2d = ctx.norms_l2(dx, dy)
+    ref = [np.abs(x).max() if n else 0.0, np.abs(x - y).max() if n else 0.0,
+           np.linalg.norm(x), np.linalg.norm(x - y)]
+    for got, want, tol in ((mi, ref[0], 1e-15), (md, ref[1], 1e-15), (l2, ref[2], 1e-13), (l2d, ref[3], 1e-13)):
+        assert abs(got - want) <= tol * max(want, 1e-300), (got, want)
+    assert ctx.norms_l2(dx, dy) == (l2, l2d)
+    assert ctx.norms_l2(dx)[1] == 0.0
+
+
 # ------------------------------------------------------------------ errors
 def test_bad_arguments_raise(H, ctx):
     with pytest.raises(H.HHError):
