@@ -57,8 +57,8 @@ def parse():
     ap.add_argument("--problems", type=int, default=512, help="uneven: number of problems")
     ap.add_argument("--sweeps", type=int, default=2, help="uneven: fixed refinement sweeps")
     ap.add_argument("--tune", action="store_true",
-                    help="uneven: autotune each group's tridiagonal shape before capture (eager timing; "
-                         "the built-in shapes measured faster in the graph)")
+                    help="uneven: autotune each group's tridiagonal shape and spectrum layout before capture, "
+                         "timing each candidate as a graph replay of the group's solve")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="after the run, copy MIB MiB three times with an elementwise kernel (y = x * 1), the "
                          "known-size reference for scaling HBM counters")
@@ -426,12 +426,14 @@ def run_uneven(args, torch, H):
         solvesmx.append((pmx, F, torch.empty_like(F)))
         shapes = {}
         if args.tune:
-            # the autotuner picks each group's tridiagonal shape (row-major spectra), before capture
+            # the autotuner picks each group's tridiagonal shape and spectrum layout, timing every
+            # candidate (the built-in one included) as a graph replay, before capture
             from hackathon_amd import autotune
             t_tune = time.perf_counter()
             for name, plan, buf in (("fp64", p64, solves64[-1][2]), ("mixed", pmx, solvesmx[-1][2])):
-                best = autotune.tune(plan, F, buf, reps=2, tiles=(-1,))
-                shapes[name] = {"W": best["W"], "rows": best["rows"], "rhs": best["rhs"], "us": round(best["us"], 1)}
+                best = autotune.tune(plan, F, buf, reps=3, tiles=(0, -1), graph=True)
+                shapes[name] = {"W": best["W"], "rows": best["rows"], "rhs": best["rhs"], "tile": best["tile"],
+                                "us": round(best["us"], 1)}
             log(f"uneven: tuned N={N} order={order} in {time.perf_counter() - t_tune:.1f} s: {shapes}")
         info.append({"N": N, "order": order, "problems": B, "k": ks, "ops": ops, "u": u_ex, "shapes": shapes})
         total_dof += B * N * N

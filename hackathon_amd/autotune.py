@@ -7,7 +7,11 @@ device with the library's own kernel timers (HIP events on the launch
 stream).  Results are kept in a JSON table keyed by
 (dimension, system length, modes, batch bucket, precision) so that a shape
 does not depend on what else a batch contains; `apply_tuned(plan, batch)`
-sets the tabulated shape on a plan.  The DST variants (half-length or
+sets the tabulated shape on a plan.  With graph=True each candidate is
+instead recorded into a HIP graph of the solve and its replays are timed
+(HIP events around the launches), which is what a graph-captured batch
+runs: launch gaps and the order of the kernels in the replay count, and the
+eager per-kernel timers do not see them.  The DST variants (half-length or
 odd-extension transform, 256-, 64- or, for strided lines, 512-thread
 blocks, and the automatic choice) are timed as well.
 
@@ -66,10 +70,29 @@ def time_tri(plan, rhs, out, reps=3):
     return 1e3 * sum(ms) / reps, [ms[k] / max(nl[k], 1) for k in range(5)]
 
 
-def tune(plan, rhs, out, candidates=CANDIDATES, reps=3, verbose=False, tiles=None):
+def time_graph(plan, rhs, out, reps=3):
+    """(microseconds of one replay of a graph holding the solve of rhs,
+    [None] * 5): the solve recorded once, replayed after one warm-up."""
+    import torch
+    g = H.Graph(plan.ctx, [(plan, rhs, out)])
+    try:
+        g.launch()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            g.launch()
+        e.record()
+        e.synchronize()
+        return 1e3 * s.elapsed_time(e) / reps, [None] * 5
+    finally:
+        g.close()
+
+
+def tune(plan, rhs, out, candidates=CANDIDATES, reps=3, verbose=False, tiles=None, graph=False):
     """Try the built-in shape and every candidate shape (and, in 2D,
     spectrum layout, or only the given tile widths) that fits; leave the
-    fastest, by kernel time of the whole solve, set on the plan."""
+    fastest set on the plan: by kernel time of the whole solve, or, with
+    graph=True, by the time of a graph replay of it."""
     lib = H.lib()
     results = []
     batch = plan._batch_of(rhs)
@@ -85,13 +108,15 @@ def tune(plan, rhs, out, candidates=CANDIDATES, reps=3, verbose=False, tiles=Non
                 continue
             H._check(lib.hh_plan_set_tile_width(plan.h, tw))
             try:
-                t, per = time_tri(plan, rhs, out, reps)
+                t, per = (time_graph if graph else time_tri)(plan, rhs, out, reps)
             except H.HHError:
                 continue
-            results.append((t, w, m, r, tw, per[0] * 1e3, per[2] * 1e3))
+            dst_us = per[0] * 1e3 if per[0] is not None else None
+            tri_us = per[2] * 1e3 if per[2] is not None else None
+            results.append((t, w, m, r, tw, dst_us, tri_us))
             if verbose:
-                print(f"  W={w:3d} rows={m:3d} rhs={r} tile={tw:3d}: solve {t:9.1f} us "
-                      f"(DST {1e3 * per[0]:7.1f}, tridiagonal {1e3 * per[2]:7.1f} us per launch)", flush=True)
+                detail = "graph replay" if graph else f"DST {dst_us:7.1f}, tridiagonal {tri_us:7.1f} us per launch"
+                print(f"  W={w:3d} rows={m:3d} rhs={r} tile={tw:3d}: solve {t:9.1f} us ({detail})", flush=True)
     results.sort()
     best = results[0]
     lib.hh_plan_set_tri_shape(plan.h, best[1], best[2], best[3])
@@ -122,6 +147,7 @@ def main():
     ap.add_argument("--precision", type=int, default=H.PREC_F64)
     ap.add_argument("--k", type=float, default=50.0)
     ap.add_argument("--save", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="time each candidate as a graph replay of the solve")
     args = ap.parse_args()
     ctx = H.Context(0)
     N = args.N
@@ -138,7 +164,9 @@ def main():
     rhs = torch.randn(shape, dtype=torch.complex128, device="cuda", generator=g)
     out = torch.empty_like(rhs)
     print(f"tridiagonal shapes, dim={args.dim} N={N} batch={args.batch} precision={args.precision}", flush=True)
-    res = tune(plan, rhs, out, verbose=True)
+    if args.precision == H.PREC_MIXED:
+        plan.set_refine(0.0, 2)   # fixed sweeps: graph-capturable, and the same work per timing
+    res = tune(plan, rhs, out, verbose=True, graph=args.graph)
     print(json.dumps({k: v for k, v in res.items() if k != "all"}), flush=True)
     # -1 automatic; bit 0 odd extension, bit 1 one-wave blocks, bit 2 512-thread
     # blocks for strided lines (N+1 in 128..256)
