@@ -743,13 +743,18 @@ static void eigenvalues(int N, std::vector<double> &v) {
 static int ensure_dev(hh_plan *p, void **ptr, size_t *cap, size_t bytes) {
     if (*ptr && bytes <= *cap) return HH_OK;
     hh_context *c = p->ctx;
+    const size_t old = *ptr ? *cap : 0;
     if (*ptr) {
         HIPCHK(hipStreamSynchronize(c->stream));
         dev_free(c, *ptr, *cap);
         *ptr = nullptr;
         *cap = 0;
     }
-    const size_t b = bytes ? bytes : 16;
+    // grow by at least half again (a serving loop's operators flag a few
+    // modes more or fewer from batch to batch: reallocate seldom, since a
+    // free waits for the stream)
+    size_t b = bytes ? bytes : 16;
+    if (old) b = std::max(b, old + old / 2);
     int rc = dev_alloc(c, ptr, b);
     if (rc) return rc;
     *cap = b;

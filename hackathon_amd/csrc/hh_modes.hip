@@ -214,9 +214,33 @@ void classify_modes(int dim, int Nx, int Ny, int Nz, int nop, const cd *hcoef, c
             double margin = INFINITY, rowsum = 0;
             const int qend = uniform ? std::min(n, 3) : n;
             for (int q = 0; q < qend; q++) {
-                RowBands b = row_bands(dim, tab + (long)q * nc, lx, ly, lxy);
-                const double s = q > 0 ? std::abs(b.sub) : 0.0, u = q < n - 1 ? std::abs(b.sup) : 0.0;
-                const double d = std::abs(b.dia);
+                // magnitudes as sqrt(re^2 + im^2) (the bands are far from
+                // overflow; std::abs's hypot costs several times more, and a
+                // serving loop classifies every request's operator)
+                double s, d, u;
+                if (real) {
+                    const cd *row = tab + (long)q * nc;
+                    double sub, dia, sup;
+                    if (dim == 2) {
+                        sub = row[0].real() * lx + row[1].real();
+                        dia = row[2].real() * lx + row[3].real();
+                        sup = row[4].real() * lx + row[5].real();
+                    } else {
+                        auto e = [&](const cd *c) {
+                            return c[0].real() * lxy + c[1].real() * lx + c[2].real() * ly + c[3].real();
+                        };
+                        dia = e(row);
+                        sup = e(row + 4);
+                        sub = e(row + 8);
+                    }
+                    s = std::fabs(sub); d = std::fabs(dia); u = std::fabs(sup);
+                } else {
+                    const RowBands b = row_bands(dim, tab + (long)q * nc, lx, ly, lxy);
+                    auto mag = [](cd z) { return std::sqrt(z.real() * z.real() + z.imag() * z.imag()); };
+                    s = mag(b.sub); d = mag(b.dia); u = mag(b.sup);
+                }
+                if (q == 0) s = 0.0;
+                if (q == n - 1) u = 0.0;
                 margin = std::min(margin, d - s - u);
                 rowsum = std::max(rowsum, d + s + u);
             }

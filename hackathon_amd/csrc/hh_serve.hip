@@ -226,7 +226,7 @@ void lane_main(hh_server *S, int lane) {
         }
         const Clock::time_point t0 = Clock::now();
         for (Req *q : batch) q->t_start = t0;
-        const int nb = (int)batch.size();
+        int nb = (int)batch.size();
         Resident &R = cache[sh];
         int rc = HH_OK;
         // grow the resident grid to nb right-hand sides
@@ -262,12 +262,36 @@ void lane_main(hh_server *S, int lane) {
             }
         }
         if (rc == HH_OK) rc = hh_plan_set_operators(R.plan, nb, coef.data());
+        // a request whose operator is singular to working precision (k^2 at
+        // an eigenvalue) fails alone: the plan's classification of the
+        // batch's operators names it, the others are solved without it
+        std::vector<Req *> refused;
+        if (rc == HH_OK) {
+            const size_t per = coef.size() / nb;
+            std::vector<Req *> keep;
+            std::vector<double> kc;
+            for (int b = 0; b < nb; b++) {
+                int sg = 0;
+                hh_plan_mode_info(R.plan, b, nullptr, &sg, nullptr);
+                if (sg) {
+                    refused.push_back(batch[b]);
+                } else {
+                    keep.push_back(batch[b]);
+                    kc.insert(kc.end(), coef.begin() + b * per, coef.begin() + (b + 1) * per);
+                }
+            }
+            if (!refused.empty()) {
+                batch.swap(keep);
+                nb = (int)batch.size();
+                if (nb > 0) rc = hh_plan_set_operators(R.plan, nb, kc.data());
+            }
+        }
         hipStream_t st = (hipStream_t)hh_context_get_stream(ctx);
         for (int b = 0; rc == HH_OK && b < nb; b++)
             if (hipMemcpyAsync((char *)R.d_rhs + b * field_bytes(sh), batch[b]->r.h_rhs, field_bytes(sh),
                                hipMemcpyHostToDevice, st) != hipSuccess)
                 rc = HH_ERR_DEVICE;
-        if (rc == HH_OK) rc = hh_solve(R.plan, R.d_rhs, R.d_sol, nb);
+        if (rc == HH_OK && nb > 0) rc = hh_solve(R.plan, R.d_rhs, R.d_sol, nb);
         for (int b = 0; rc == HH_OK && b < nb; b++)
             if (hipMemcpyAsync(batch[b]->r.h_sol, (char *)R.d_sol + b * field_bytes(sh), field_bytes(sh),
                                hipMemcpyDeviceToHost, st) != hipSuccess)
@@ -283,6 +307,11 @@ void lane_main(hh_server *S, int lane) {
                 q->status = rc;
                 q->sweeps = sweeps;
                 q->rel = rel;
+                q->t_done = t1;
+                q->done = true;
+            }
+            for (Req *q : refused) {
+                q->status = HH_ERR_SINGULAR;
                 q->t_done = t1;
                 q->done = true;
             }
@@ -325,27 +354,10 @@ extern "C" int hh_server_submit(hh_server *S, const hh_request *r, long *ticket)
     q->r = *r;
     q->shape = Shape{r->dim, r->Nx, r->Ny, r->dim == 3 ? r->Nz : 1, r->precision};
     q->t_arrive = Clock::now();
-    // a request whose operator is singular to working precision (k^2 at an
-    // eigenvalue) fails alone, before it reaches a batch
-    std::vector<double> coef;
-    int sing = 0;
-    int rc = request_coef(*r, coef);
-    if (rc == HH_OK)
-        rc = hh_operator_modes(r->dim, r->Nx, r->Ny, r->dim == 3 ? r->Nz : 1, coef.data(), 4096.0, nullptr, nullptr,
-                               0, &sing, nullptr);
-    if (rc == HH_OK && sing) rc = HH_ERR_SINGULAR;
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->stop) return HH_ERR_ARG;
     q->id = S->next_id++;
     *ticket = q->id;
-    if (rc != HH_OK) {
-        q->status = rc;
-        q->done = true;
-        q->t_start = q->t_done = Clock::now();
-        S->reqs[q->id] = std::move(q);
-        S->cv_done.notify_all();
-        return HH_OK;
-    }
     S->queue.push_back(q.get());
     S->reqs[q->id] = std::move(q);
     S->cv_work.notify_all();

@@ -328,7 +328,10 @@ int hh_norms_l2(hh_context *ctx, long n, const void *d_x, const void *d_y, doubl
  * One request: constant wavenumber k on the grid with spacings h*, compact
  * scheme of the given order, one right-hand side in host memory (complex
  * interleaved, row-major); the solution is written to h_sol.  Both buffers
- * must stay valid until hh_server_wait returns for the ticket. */
+ * must stay valid until hh_server_wait returns for the ticket.  A request
+ * whose operator is singular to working precision (k^2 at a discrete
+ * eigenvalue) completes with HH_ERR_SINGULAR and h_sol untouched; the
+ * requests batched with it are solved without it. */
 typedef struct hh_server hh_server;
 typedef struct {
     int dim, Nx, Ny, Nz;      /* Nz ignored in 2D */

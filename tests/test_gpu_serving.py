@@ -90,3 +90,28 @@ def test_serving_budget_refuses_oversized(H):
     sol, _ = srv.wait(srv.submit(2, 31, 4, 10.0, small))
     assert np.isfinite(sol).all()
     srv.close()
+
+
+def test_singular_request_fails_alone(H):
+    """A request with k^2 at a discrete eigenvalue of its grid completes with
+    HH_ERR_SINGULAR and its output untouched; requests of the same grid
+    batched with it give the bits they give alone."""
+    N = 63
+    h = 1.0 / (N + 1)
+    i, j = 1, 4
+    k2 = (4 - 2 * np.cos((i + 1) * np.pi / (N + 1)) - 2 * np.cos((j + 1) * np.pi / (N + 1))) / h ** 2
+    rng = np.random.default_rng(99)
+    rhs = [rng.standard_normal(N * N) + 1j * rng.standard_normal(N * N) for _ in range(3)]
+    ks = [9.5, float(np.sqrt(k2)), 21.25]
+    srv = H.Server(0, lanes=1, max_batch=8)
+    alone = [srv.wait(srv.submit(2, N, 2, k, r))[0] for k, r in ((ks[0], rhs[0]), (ks[2], rhs[2]))]
+    outs = [np.full(N * N, 7.0 + 0j) for _ in range(3)]
+    tickets = [srv.submit(2, N, 2, k, r, out=o) for k, r, o in zip(ks, rhs, outs)]
+    say("singular request among two regular ones, one lane")
+    assert np.array_equal(srv.wait(tickets[0])[0], alone[0])
+    with pytest.raises(H.HHError) as ei:
+        srv.wait(tickets[1])
+    assert ei.value.status == H.HH_ERR_SINGULAR
+    assert bool((outs[1] == 7.0).all())
+    assert np.array_equal(srv.wait(tickets[2])[0], alone[1])
+    srv.close()
