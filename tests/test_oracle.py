@@ -131,3 +131,22 @@ def test_planted_stencil_error_is_caught(oracle):
     bad[0] = bad[0] * 0  # drop the corner weights bp
     sol = oracle.solver2d(Nx, Ny, s["rhs"], oracle.lu2d(Nx, Ny, bad))
     assert np.abs(sol - ref).max() > 1e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("N", [63, 100])
+def test_planned_solvers_equal_unplanned(oracle, N):
+    """The planned solvers (the DST plan created once by the caller, as in the
+    steady-state CPU baseline) give the same bits as the solvers that plan
+    inside the call, over several reuses of one plan; 2D and 3D, FFT and
+    Bluestein lengths."""
+    s = oracle.solve2d(6, N, 1, [10.0, 0.0, 3, 2, 0.3], threads=2)
+    plan = oracle.DstPlan(N)
+    for _ in range(2):
+        assert np.array_equal(oracle.solver2d_planned(plan, N, N, s["rhs"], s["LU"], 2), s["sol"])
+    n3 = 15 if N == 63 else 20
+    t = oracle.setup3d(6, n3, n3, n3, oracle.SINE, [10.0, 0.0, 1, 2, 3, 1.0, 1.0], threads=2)
+    LUU = oracle.lu3d(n3, n3, n3, t["coef"])
+    ref = oracle.solver3d(n3, n3, n3, t["rhs"], LUU, 2)
+    p3 = oracle.DstPlan(n3)
+    for _ in range(2):
+        assert np.array_equal(oracle.solver3d_planned(p3, p3, n3, n3, n3, t["rhs"], LUU, 2), ref)
