@@ -101,6 +101,7 @@ struct hh_server {
     std::atomic<int> lane_error{HH_OK};
     std::vector<std::map<Shape, Resident>> resident;   // per lane
     std::vector<char> busy;                             // lane is running a batch
+    std::vector<size_t> lane_ws;                        // workspace charged to each lane
 };
 
 namespace {
@@ -128,17 +129,27 @@ int request_coef(const hh_request &r, std::vector<double> &out) {
     return hh_coefficients3d(r.order, r.Nx, r.Ny, r.Nz, r.hx, r.hy, r.hz, k2e.data(), nullptr, v);
 }
 
-// Device bytes a resident grid needs for cap right-hand sides: buffers,
-// coefficient tables and eigenvalues of max_batch operators, and the
-// solve's workspace (estimated; refined once the plan exists).
+// Device bytes a resident grid needs for cap right-hand sides: its buffers
+// and the coefficient tables and eigenvalues of max_batch operators.
 size_t resident_bytes(const Shape &s, int cap, int max_batch) {
     // coefficient tables per entry: complex FP64 and FP32 (16 + 8 B) and
     // their real parts (8 + 4 B) for the real-band kernels
     const size_t tables = (size_t)max_batch * (s.dim == 2 ? 6 : 12) * s.sys() * 36 + 8 * (s.Nx + s.Ny);
-    // context scratch of a solve: the 2D mode-tiled spectrum (16 B per DOF in
-    // FP64, 8 B in FP32 plus the 8 B FP32 correction), the 3D FP32 correction
-    const size_t ws = (size_t)cap * s.field() * (s.dim == 2 ? 16 : (s.prec == HH_PREC_MIXED ? 8 : 0));
-    return 2 * (size_t)cap * field_bytes(s) + tables + ws;
+    return 2 * (size_t)cap * field_bytes(s) + tables;
+}
+
+// Workspace of one solve of cap right-hand sides on the server's plans
+// (row-major spectra), as hh_plan_workspace_bytes states it: the dense
+// DST's ping-pong field (16 B/DOF) unless every transformed length has N+1
+// a power of two up to 4096 (the FFT path), and the FP32 correction of
+// mixed precision (8 B/DOF).  It lives in the lane context's scratch,
+// which keeps the largest size any batch needed until the lane ends, so it
+// is charged to the lane, not to a resident grid.
+size_t solve_ws_bytes(const Shape &s, int cap) {
+    auto fft = [](int N) { const int m = N + 1; return m >= 2 && m <= 4096 && (m & (m - 1)) == 0; };
+    const bool all_fft = fft(s.Nx) && (s.dim == 2 || fft(s.Ny));
+    const size_t f = (size_t)cap * s.field();
+    return (all_fft ? 0 : 16 * f) + (s.prec == HH_PREC_MIXED ? 8 * f : 0);
 }
 
 void release(hh_server *S, Resident &R) {
@@ -163,12 +174,14 @@ void lane_main(hh_server *S, int lane) {
     for (;;) {
         std::vector<Req *> batch;
         Shape sh{};
-        size_t grow = 0;
+        size_t grow = 0, wgrow = 0;
         {
             std::unique_lock<std::mutex> lk(S->mu);
             for (;;) {
                 if (S->stop && S->queue.empty()) {
                     for (auto &kv : cache) release(S, kv.second);
+                    S->reserved -= std::min(S->reserved, S->lane_ws[lane]);
+                    S->lane_ws[lane] = 0;
                     lk.unlock();
                     hh_context_destroy(ctx);
                     return;
@@ -182,10 +195,12 @@ void lane_main(hh_server *S, int lane) {
                     Resident &R = cache[sh];
                     const size_t need = resident_bytes(sh, std::max(nb, R.cap), S->max_batch);
                     grow = need > R.bytes ? need - R.bytes : 0;
+                    const size_t wneed = solve_ws_bytes(sh, nb);
+                    wgrow = wneed > S->lane_ws[lane] ? wneed - S->lane_ws[lane] : 0;
                     // make room: drop the least recently used resident grids
                     // of idle lanes (this one included, except the grid it
                     // is about to use)
-                    while (S->budget && S->reserved + grow > S->budget) {
+                    while (S->budget && S->reserved + grow + wgrow > S->budget) {
                         Resident *victim = nullptr;
                         for (int l = 0; l < S->lanes; l++) {
                             if (l != lane && S->busy[l]) continue;
@@ -197,12 +212,13 @@ void lane_main(hh_server *S, int lane) {
                         if (!victim) break;
                         release(S, *victim);
                     }
-                    if (!S->budget || S->reserved + grow <= S->budget) {
+                    if (!S->budget || S->reserved + grow + wgrow <= S->budget) {
                         for (auto it = S->queue.begin(); it != S->queue.end() && (int)batch.size() < nb;) {
                             if ((*it)->shape == sh) { batch.push_back(*it); it = S->queue.erase(it); }
                             else ++it;
                         }
-                        S->reserved += grow;
+                        S->reserved += grow + wgrow;
+                        S->lane_ws[lane] += wgrow;
                         S->busy[lane] = 1;
                         R.last_use = ++S->clock;
                         break;
@@ -339,6 +355,7 @@ extern "C" int hh_server_create(int device, size_t hbm_budget, int lanes, int ma
     S->max_batch = max_batch;
     S->resident.resize(lanes);
     S->busy.assign(lanes, 0);
+    S->lane_ws.assign(lanes, 0);
     for (int l = 0; l < lanes; l++) S->threads.emplace_back(lane_main, S, l);
     *out = S;
     return HH_OK;

@@ -115,3 +115,18 @@ def test_singular_request_fails_alone(H):
     assert bool((outs[1] == 7.0).all())
     assert np.array_equal(srv.wait(tickets[2])[0], alone[1])
     srv.close()
+
+
+def test_budget_counts_lane_workspace(H):
+    """The HBM budget charges a lane for its solve workspace: a 3D FP64 grid
+    with N+1 not a power of two (dense DST, 16 B per DOF of ping-pong field)
+    reserves its two buffers plus that workspace."""
+    N = 20
+    srv = H.Server(0, hbm_budget=1 << 30, lanes=1, max_batch=1)
+    rhs = np.ones(N ** 3, np.complex128)
+    sol, _ = srv.wait(srv.submit(3, N, 6, 5.0, rhs))
+    assert np.isfinite(sol).all()
+    field = N ** 3 * 16
+    say(f"reserved {srv.stats()['reserved_bytes']} B for a {field} B field")
+    assert srv.stats()["reserved_bytes"] >= 3 * field
+    srv.close()
