@@ -184,6 +184,13 @@ template <typename T, typename Ti, typename To, int DIM, bool REAL> struct HelmP
     HH_DEV void store64(int b, const Mode &md, int q, double2 x) const {
         out[b * field + md.off + (long)q * md.step] = cconv<To, double>(x);
     }
+    // one part (comp 0 real, 1 imaginary) of an element, as FP64
+    HH_DEV double rhs64c(int b, const Mode &md, int q, int comp) const {
+        return (double)reinterpret_cast<const Ti *>(in + b * field + md.off + (long)q * md.step)[comp];
+    }
+    HH_DEV void store64c(int b, const Mode &md, int q, int comp, double v) const {
+        reinterpret_cast<To *>(out + b * field + md.off + (long)q * md.step)[comp] = (To)v;
+    }
 };
 
 // Compact fourth-order derivative of a field sampled on the extended grid
@@ -489,6 +496,8 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
 // right-hand side b, its operator's near-resonant entry e_o.
 template <typename OT> struct SeqOps;
 template <> struct SeqOps<double> {   // real factors, complex right-hand sides
+    static HH_DEV double zero() { return 0.0; }
+    static HH_DEV double one() { return 1.0; }
     static HH_DEV double2 fms(double a, double2 x, double2 y) {   // y - a x
 #pragma clang fp contract(off)
         return mk<double>(y.x - a * x.x, y.y - a * x.y);
@@ -501,6 +510,8 @@ template <> struct SeqOps<double> {   // real factors, complex right-hand sides
     static HH_DEV double2 div(double2 a, double p, double r) { return mk<double>(q1(a.x, p, r), q1(a.y, p, r)); }
 };
 template <> struct SeqOps<double2> {   // complex factors: C99 products, quotients through the reciprocal
+    static HH_DEV double2 zero() { return mk<double>(0, 0); }
+    static HH_DEV double2 one() { return mk<double>(1, 0); }
     static HH_DEV double2 mul(double2 a, double2 x) {
 #pragma clang fp contract(off)
         return mk<double>(a.x * x.x - a.y * x.y, a.x * x.y + a.y * x.x);
@@ -533,17 +544,24 @@ __global__ __launch_bounds__(512) void k_tri_seq(Prob prob, const int *foff, con
     // idle chains of the last block run on a valid item and store nothing
     const int bb = live ? b : 0, e = live ? foff[o] + x : 0;
     const auto md = prob.mode(fmode[e], bb);
+    // Rows past n (the tail of the last position) become identity rows
+    // (d = 0, L = 0, P = R = 1, Up = 0): the forward walk ends on them with a
+    // carry nobody reads, the backward walk leaves them with 0, as it starts,
+    // so every real row keeps its bits, and the walks need no per-row branch.
     double2 d[R];
     OT L[R], Pv[R], Rv[R], Up[R];
 #pragma unroll
     for (int i = 0; i < R; i++) {
         const int q = min(s + i, n - 1);
-        d[i] = prob.rhs64(bb, md, q);
+        const bool pad = s + i >= n;
+        const double2 v = prob.rhs64(bb, md, q);
         const OT *f = fac + ((long)q * nent + e) * 4;
-        L[i] = f[0];
-        Pv[i] = f[1];
-        Rv[i] = f[2];
-        Up[i] = f[3];
+        const OT f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3];
+        d[i] = pad ? mk<double>(0, 0) : v;
+        L[i] = pad ? S::zero() : f0;
+        Pv[i] = pad ? S::one() : f1;
+        Rv[i] = pad ? S::one() : f2;
+        Up[i] = pad ? S::zero() : f3;
     }
     // forward: y_q = d_q - L_q y_{q-1} (L_0 = 0)
     double2 cy = mk<double>(0, 0);
@@ -553,11 +571,10 @@ __global__ __launch_bounds__(512) void k_tri_seq(Prob prob, const int *foff, con
             for (int pp = 0; pp < PW; pp++) {
                 if (pw == pp) {
 #pragma unroll
-                    for (int i = 0; i < R; i++)
-                        if (s + i < n) {
-                            d[i] = S::fms(L[i], cy, d[i]);
-                            cy = d[i];
-                        }
+                    for (int i = 0; i < R; i++) {
+                        d[i] = S::fms(L[i], cy, d[i]);
+                        cy = d[i];
+                    }
                     if (pp == PW - 1) carry[c] = cy;
                 }
                 if (pp < PW - 1) cy = shfl_up2(cy, C);
@@ -573,13 +590,12 @@ __global__ __launch_bounds__(512) void k_tri_seq(Prob prob, const int *foff, con
             for (int pp = PW - 1; pp >= 0; pp--) {
                 if (pw == pp) {
 #pragma unroll
-                    for (int i = R - 1; i >= 0; i--)
-                        if (s + i < n) {
-                            const double2 a = S::fms(Up[i], cy, d[i]);
-                            if constexpr (DIV) d[i] = S::div(a, Pv[i], Rv[i]);
-                            else d[i] = S::mul(Rv[i], a);
-                            cy = d[i];
-                        }
+                    for (int i = R - 1; i >= 0; i--) {
+                        const double2 a = S::fms(Up[i], cy, d[i]);
+                        if constexpr (DIV) d[i] = S::div(a, Pv[i], Rv[i]);
+                        else d[i] = S::mul(Rv[i], a);
+                        cy = d[i];
+                    }
                     if (pp == 0) carry[c] = cy;
                 }
                 if (pp > 0) cy = shfl_down2(cy, C);
@@ -591,6 +607,92 @@ __global__ __launch_bounds__(512) void k_tri_seq(Prob prob, const int *foff, con
 #pragma unroll
     for (int i = 0; i < R; i++)
         if (s + i < n) prob.store64(b, md, s + i, d[i]);
+}
+
+// Real factors: the real and imaginary parts of a right-hand side follow
+// two independent real recurrences with the same factors, so neighbouring
+// lanes run them as separate chains (chain = 2 item + part): half the
+// instructions per row on the walk's critical path, each operation the one
+// k_tri_seq applies to that part (same bits).
+HH_DEV double fms1(double a, double x, double y) {   // y - a x, unfused
+#pragma clang fp contract(off)
+    return y - a * x;
+}
+
+template <int R, int C, bool DIV, class Prob>
+__global__ __launch_bounds__(512) void k_tri_seq_re(Prob prob, const int *foff, const int *fmode, const double *fac,
+                                                    long nent, int nfx, int batch, int nop) {
+    constexpr int PW = 64 / C;   // positions per wave
+    __shared__ double carry[C];
+    const int n = prob.n();
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6, NW = blockDim.x >> 6;
+    const int c = lane % C, pw = lane / C;
+    const int s = (wave * PW + pw) * R;   // first row of this lane
+    const long chain = (long)blockIdx.x * C + c;
+    const long item = chain >> 1;
+    const int comp = (int)(chain & 1);
+    const int b = (int)(item / nfx), x = (int)(item % nfx);
+    const int o = b < batch ? b % nop : 0;
+    const bool live = b < batch && x < foff[o + 1] - foff[o];
+    const int bb = live ? b : 0, e = live ? foff[o] + x : 0;
+    const auto md = prob.mode(fmode[e], bb);
+    // rows past n: identity rows, as in k_tri_seq
+    double d[R], L[R], Pv[R], Rv[R], Up[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        const int q = min(s + i, n - 1);
+        const bool pad = s + i >= n;
+        const double v = prob.rhs64c(bb, md, q, comp);
+        const double *f = fac + ((long)q * nent + e) * 4;
+        const double f0 = f[0], f1 = f[1], f2 = f[2], f3 = f[3];
+        d[i] = pad ? 0.0 : v;
+        L[i] = pad ? 0.0 : f0;
+        Pv[i] = pad ? 1.0 : f1;
+        Rv[i] = pad ? 1.0 : f2;
+        Up[i] = pad ? 0.0 : f3;
+    }
+    double cy = 0.0;
+    for (int w = 0; w < NW; w++) {
+        if (wave == w) {
+            if (w > 0) cy = carry[c];
+            for (int pp = 0; pp < PW; pp++) {
+                if (pw == pp) {
+#pragma unroll
+                    for (int i = 0; i < R; i++) {
+                        d[i] = fms1(L[i], cy, d[i]);
+                        cy = d[i];
+                    }
+                    if (pp == PW - 1) carry[c] = cy;
+                }
+                if (pp < PW - 1) cy = __shfl_up(cy, C, 64);
+            }
+        }
+        __syncthreads();
+    }
+    cy = 0.0;
+    for (int w = NW - 1; w >= 0; w--) {
+        if (wave == w) {
+            if (w < NW - 1) cy = carry[c];
+            for (int pp = PW - 1; pp >= 0; pp--) {
+                if (pw == pp) {
+#pragma unroll
+                    for (int i = R - 1; i >= 0; i--) {
+                        const double a = fms1(Up[i], cy, d[i]);
+                        if constexpr (DIV) d[i] = SeqOps<double>::q1(a, Pv[i], Rv[i]);
+                        else d[i] = Rv[i] * a;
+                        cy = d[i];
+                    }
+                    if (pp == 0) carry[c] = cy;
+                }
+                if (pp > 0) cy = __shfl_down(cy, C, 64);
+            }
+        }
+        __syncthreads();
+    }
+    if (!live) return;
+#pragma unroll
+    for (int i = 0; i < R; i++)
+        if (s + i < n) prob.store64c(b, md, s + i, comp, d[i]);
 }
 
 template <typename T, int MM, int NR, class Prob>
@@ -738,10 +840,28 @@ static hipError_t launch_seq_rc(const TriHelm &t, const void *in, void *out, hip
     const int NT = G * C;
     if (NT > 512) return hipErrorInvalidValue;
     const long items = (long)t.nfx * t.batch;
-    const long blocks = (items + C - 1) / C;
-    if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_tri_seq<R, C, DIM == 2, HelmProb<double, Ti, To, DIM, REAL>, OT>), dim3((unsigned)blocks),
-                       dim3(NT), 0, st, pb, t.foff, t.fmode, (const OT *)t.fac, t.nent, t.nfx, t.batch, pb.bands.nop);
+    // Real factors: one chain per part (k_tri_seq_re) when the doubled grid
+    // still runs as one wave of blocks, one block per CU (a block holds a
+    // CU's registers); otherwise one chain per item.  C2's 1152 items: the
+    // split grid's 288 blocks ran in two rounds, 79.6 against 64 us.
+    int ncu = 0;
+    if constexpr (REAL) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            ncu = 0;
+    }
+    if (REAL && (2 * items + C - 1) / C <= ncu) {   // two chains per item: its real and imaginary parts
+        const long blocks = (2 * items + C - 1) / C;
+        hipLaunchKernelGGL((k_tri_seq_re<R, C, DIM == 2, HelmProb<double, Ti, To, DIM, REAL>>), dim3((unsigned)blocks),
+                           dim3(NT), 0, st, pb, t.foff, t.fmode, (const double *)t.fac, t.nent, t.nfx, t.batch,
+                           pb.bands.nop);
+    } else {
+        const long blocks = (items + C - 1) / C;
+        if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_tri_seq<R, C, DIM == 2, HelmProb<double, Ti, To, DIM, REAL>, OT>), dim3((unsigned)blocks),
+                           dim3(NT), 0, st, pb, t.foff, t.fmode, (const OT *)t.fac, t.nent, t.nfx, t.batch, pb.bands.nop);
+    }
     return hipGetLastError();
 }
 

@@ -139,3 +139,22 @@ def test_singular_operator_refused(H, ctx):
         plan.solve(rhs, out)
     assert ei.value.status == H.HH_ERR_SINGULAR
     assert bool((out == 7.0).all())
+
+
+def test_chain_forms_same_bits(H, ctx):
+    """The near-resonant modes' substitution runs one chain per right-hand
+    side when the batch is large (N=1023 x 64: 1152 chains) and one chain per
+    real or imaginary part when it is small (one right-hand side): the same
+    operations on each part, so the same bits."""
+    import torch
+    N, k = 1023, 50.0
+    h = 1.0 / (N + 1)
+    coef = H.coefficients2d(6, N, N, h, h, np.full(N + 2, k * k, np.complex128))
+    plan = H.Plan2D(ctx, N, N, coef)
+    assert plan.mode_info()[0] > 0
+    g = torch.Generator(device="cuda").manual_seed(17)
+    rhs = torch.randn((64, N, N), dtype=torch.complex128, device="cuda", generator=g)
+    say("chain forms: N=1023 x 64 against the first right-hand side alone")
+    many = plan.solve(rhs)[0].cpu().numpy()
+    one = plan.solve(rhs[:1].contiguous())[0].cpu().numpy()
+    assert np.array_equal(many, one)
