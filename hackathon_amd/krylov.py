@@ -25,14 +25,21 @@ def gmres(apply_a, b, precond=None, tol=1e-10, restart=30, maxiter=300):
     apply_a(v) -> A v and precond(v) -> M^{-1} v take and return complex128
     GPU tensors of b's shape.  Stops when ||b - A x||_2 <= tol ||b||_2 (the
     true residual, recomputed at every restart) or after maxiter inner
-    iterations.  Returns (x, info) with info = {iterations, rel_residual,
-    converged}."""
+    iterations.  The Krylov basis is one device tensor; each new vector is
+    orthogonalised against it by classical Gram-Schmidt applied twice (two
+    matrix-vector products on the device), so an iteration reads back one
+    Hessenberg column and one norm to the host, not one inner product per
+    basis vector.  Returns (x, info) with info = {iterations,
+    rel_residual, converged}."""
     torch = H._torch()
     M = precond if precond is not None else (lambda v: v)
+    shape = b.shape
+    n = b.numel()
     nb = float(torch.linalg.vector_norm(b))
     x = torch.zeros_like(b)
     if nb == 0.0:
         return x, {"iterations": 0, "rel_residual": 0.0, "converged": True}
+    V = torch.empty((restart + 1, n), dtype=b.dtype, device=b.device)
     r = b.clone()
     rel = 1.0
     it = 0
@@ -41,7 +48,7 @@ def gmres(apply_a, b, precond=None, tol=1e-10, restart=30, maxiter=300):
         rel = beta / nb
         if rel <= tol or it >= maxiter:
             break
-        V = [r / beta]
+        V[0] = (r / beta).reshape(-1)
         Hm = np.zeros((restart + 1, restart), np.complex128)
         cs = np.zeros(restart, np.complex128)
         sn = np.zeros(restart, np.complex128)
@@ -49,12 +56,16 @@ def gmres(apply_a, b, precond=None, tol=1e-10, restart=30, maxiter=300):
         g[0] = beta
         k = 0
         for j in range(restart):
-            w = apply_a(M(V[j]))
-            for i in range(j + 1):   # modified Gram-Schmidt
-                hij = complex(torch.vdot(V[i].reshape(-1), w.reshape(-1)))
-                Hm[i, j] = hij
-                w = w - hij * V[i]
-            hn = float(torch.linalg.vector_norm(w))
+            w = apply_a(M(V[j].reshape(shape))).reshape(-1)
+            B = V[: j + 1]
+            h1 = torch.mv(B.conj(), w)           # CGS, then once more for orthogonality
+            w = w - torch.mv(B.transpose(0, 1), h1)
+            h2 = torch.mv(B.conj(), w)
+            w = w - torch.mv(B.transpose(0, 1), h2)
+            hn_t = torch.linalg.vector_norm(w)
+            col = torch.cat([h1 + h2, hn_t.to(b.dtype).reshape(1)]).cpu().numpy()   # the one read-back
+            Hm[: j + 1, j] = col[: j + 1]
+            hn = float(col[j + 1].real)
             Hm[j + 1, j] = hn
             for i in range(j):       # earlier Givens rotations on the new column
                 t = cs[i] * Hm[i, j] + sn[i] * Hm[i + 1, j]
@@ -76,11 +87,10 @@ def gmres(apply_a, b, precond=None, tol=1e-10, restart=30, maxiter=300):
             k = j + 1
             if hn == 0.0 or abs(g[j + 1]) / nb <= tol or it >= maxiter:
                 break
-            V.append(w / hn)
+            V[j + 1] = w / hn
         y = np.linalg.solve(np.triu(Hm[:k, :k]), g[:k])
-        upd = V[0] * complex(y[0])
-        for i in range(1, k):
-            upd = upd + V[i] * complex(y[i])
+        yt = torch.from_numpy(y).to(device=b.device, dtype=b.dtype)
+        upd = torch.mv(V[:k].transpose(0, 1), yt).reshape(shape)
         x = x + M(upd)
         r = b - apply_a(x)
     return x, {"iterations": it, "rel_residual": rel, "converged": rel <= tol}
@@ -121,3 +131,42 @@ class VariableK2D:
     def solve(self, F, tol=1e-12, restart=30, maxiter=300):
         """x with A x = F (F: (N, N) complex128 GPU tensor)."""
         return gmres(self.apply, F.reshape(self.N, self.N).contiguous(), self.precond, tol, restart, maxiter)
+
+
+class VariableK3D:
+    """Second-order seven-point Helmholtz operator on the unit cube with a
+    wave number varying in x, y and z (homogeneous Dirichlet boundary):
+
+        (A u)_ijl = u_{i-1} + u_{i+1} + u_{j-1} + u_{j+1} + u_{l-1} + u_{l+1}
+                    + (h^2 k^2(x_i, y_j, z_l) - 6) u_ijl
+
+    (the reference's 3D second-order stencil, 3D_Program/main_functions/
+    coefficients.c:7-32, with h = hx = hy = hz and k^2 taken per point).  k2
+    is an (N, N, N) array of k^2 at the interior points, [l][j][i] =
+    k^2(x_i, y_j, z_l).  The preconditioner is the direct solve of the same
+    stencil with k^2 averaged over x and y in each z layer (what the 3D DST
+    diagonalisation allows); A u is its residual kernel plus the diagonal
+    h^2 (k^2 - mean_xy k^2) u."""
+
+    def __init__(self, ctx, N, k2):
+        torch = H._torch()
+        k2 = np.asarray(k2, dtype=np.complex128).reshape(N, N, N)
+        self.N, self.h = N, 1.0 / (N + 1)
+        kbar = k2.mean(axis=(1, 2))                              # per layer l
+        k2e = np.concatenate([[kbar[0]], kbar, [kbar[-1]]])
+        coef = H.coefficients3d(2, N, N, N, self.h, self.h, self.h, k2e)
+        self.plan = H.Plan3D(ctx, N, N, N, coef)
+        d = self.h ** 2 * (k2 - kbar[:, None, None])
+        self.diag = torch.from_numpy(np.ascontiguousarray(d)).to("cuda")
+        self.zero = torch.zeros((N, N, N), dtype=torch.complex128, device="cuda")
+
+    def apply(self, u):
+        u = u.reshape(self.N, self.N, self.N)
+        return self.plan.residual(u, self.zero) + self.diag * u
+
+    def precond(self, v):
+        return self.plan.solve(v.reshape(self.N, self.N, self.N).contiguous())
+
+    def solve(self, F, tol=1e-12, restart=30, maxiter=300):
+        """x with A x = F (F: (N, N, N) complex128 GPU tensor)."""
+        return gmres(self.apply, F.reshape(self.N, self.N, self.N).contiguous(), self.precond, tol, restart, maxiter)
