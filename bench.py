@@ -56,6 +56,9 @@ def parse():
     ap.add_argument("--lanes", type=int, default=4, help="serve: server lanes (streams)")
     ap.add_argument("--problems", type=int, default=512, help="uneven: number of problems")
     ap.add_argument("--sweeps", type=int, default=2, help="uneven: fixed refinement sweeps")
+    ap.add_argument("--by-order", action="store_true",
+                    help="uneven: one plan per (N, order) group instead of one per N (orders 4 and 6 of one N "
+                         "share a multi-operator plan by default)")
     ap.add_argument("--tune", action="store_true",
                     help="uneven: autotune each group's tridiagonal shape and spectrum layout before capture, "
                          "timing each candidate as a graph replay of the group's solve")
@@ -375,8 +378,8 @@ def run_3d(args, torch, H):
 
 def run_uneven(args, torch, H):
     """512 uneven 2D problems (N log-uniform in 63..2047, k in [1,80], order 4/6), FP32 bulk + FP64
-    refinement with fixed sweeps, grouped by (N, order) into multi-operator plans and replayed as one
-    HIP graph."""
+    refinement with fixed sweeps, grouped by N into multi-operator plans (one operator per problem, so
+    both orders share a plan; --by-order: one plan per (N, order)) and replayed as one HIP graph."""
     dev = torch.device("cuda:0")
     rng = np.random.default_rng(args.seed)
     Ns = np.array([63, 127, 255, 511, 1023, 2047])
@@ -425,7 +428,7 @@ def run_uneven(args, torch, H):
         solves64.append((p64, F, torch.empty_like(F)))
         solvesmx.append((pmx, F, torch.empty_like(F)))
         shapes = {}
-        if args.tune:
+        if args.tune and args.by_order:
             # the autotuner picks each group's tridiagonal shape and spectrum layout, timing every
             # candidate (the built-in one included) as a graph replay, before capture
             from hackathon_amd import autotune
@@ -435,9 +438,45 @@ def run_uneven(args, torch, H):
                 shapes[name] = {"W": best["W"], "rows": best["rows"], "rhs": best["rhs"], "tile": best["tile"],
                                 "us": round(best["us"], 1)}
             log(f"uneven: tuned N={N} order={order} in {time.perf_counter() - t_tune:.1f} s: {shapes}")
-        info.append({"N": N, "order": order, "problems": B, "k": ks, "ops": ops, "u": u_ex, "shapes": shapes})
+        info.append({"N": N, "order": order, "problems": B, "k": ks, "ops": ops, "u": u_ex, "shapes": shapes,
+                     "solve": len(solves64) - 1, "off": 0})
         total_dof += B * N * N
         log(f"uneven: group N={N} order={order}: {B} problems")
+    if not args.by_order:
+        # both orders of one N in one multi-operator plan: half the launches, twice the batch
+        byN = {}
+        for gi in info:
+            byN.setdefault(gi["N"], []).append(gi)
+        m64, mmx, uu = [], [], []
+        for N, gl in sorted(byN.items()):
+            ops = [op for gi in gl for op in gi["ops"]]
+            F = torch.cat([solves64[gi["solve"]][1] for gi in gl])
+            off = 0
+            for gi in gl:
+                gi["solve"], gi["off"] = len(m64), off
+                off += gi["problems"]
+            p64 = H.Plan2D(ctx, N, N, ops)
+            pmx = H.Plan2D(ctx, N, N, ops, precision=H.PREC_MIXED)
+            pmx.set_refine(0.0, args.sweeps)
+            m64.append((p64, F, torch.empty_like(F)))
+            mmx.append((pmx, F, torch.empty_like(F)))
+            uu.append(torch.cat([gi["u"] for gi in gl]))
+        solves64, solvesmx = m64, mmx
+        exact = uu
+        if args.tune:   # each merged plan's shape and layout, timed as graph replays, before capture
+            from hackathon_amd import autotune
+            for (p64, F, b64), (pmx, _, bmx), (N, gl) in zip(m64, mmx, sorted(byN.items())):
+                t_tune = time.perf_counter()
+                shapes = {}
+                for name, plan, buf in (("fp64", p64, b64), ("mixed", pmx, bmx)):
+                    best = autotune.tune(plan, F, buf, reps=3, tiles=(0, -1), graph=True)
+                    shapes[name] = {"W": best["W"], "rows": best["rows"], "rhs": best["rhs"], "tile": best["tile"],
+                                    "us": round(best["us"], 1)}
+                for gi in gl:
+                    gi["shapes"] = shapes
+                log(f"uneven: tuned N={N} in {time.perf_counter() - t_tune:.1f} s: {shapes}")
+    else:
+        exact = [gi["u"] for gi in info]
     torch.cuda.synchronize()
     out = {}
     for name, solves in (("fp64", solves64), ("mixed", solvesmx)):
@@ -460,8 +499,8 @@ def run_uneven(args, torch, H):
                      "graph_nodes": g.num_nodes}
         # discretisation error of every problem against its exact solution
         worst = 0.0
-        for (plan, F, sol), gi in zip(solves, info):
-            mx, d = ctx.norms(sol, gi["u"])
+        for (plan, F, sol), u in zip(solves, exact):
+            mx, d = ctx.norms(sol, u)
             worst = max(worst, d / mx)
         out[name]["max_disc_err_vs_exact"] = worst
         g.close()
@@ -469,14 +508,15 @@ def run_uneven(args, torch, H):
     if not args.no_check:
         from oracle import oracle as O
         log("uneven: oracle checks")
-        for (p64, F, sol64), (pmx, _, solmx), gi in zip(solves64, solvesmx, info):
+        for gi in info:
             if gi["N"] > 1023:
                 continue
-            N = gi["N"]
+            N, i, o = gi["N"], gi["solve"], gi["off"]
+            F, sol64, solmx = solves64[i][1], solves64[i][2], solvesmx[i][2]
             LUU = O.lu2d(N, N, gi["ops"][0], 16)
-            ref = O.solver2d(N, N, F[0].cpu().numpy().ravel(), LUU, 16)
+            ref = O.solver2d(N, N, F[o].cpu().numpy().ravel(), LUU, 16)
             for name, sol in (("fp64", sol64), ("mixed", solmx)):
-                g = sol[0].cpu().numpy().ravel()
+                g = sol[o].cpu().numpy().ravel()
                 check[name][f"N{N}_order{gi['order']}"] = float(np.abs(g - ref).max() / np.abs(ref).max())
         for name in ("fp64", "mixed"):
             out[name]["max_rel_err_vs_oracle"] = max(check[name].values()) if check[name] else None
@@ -486,7 +526,8 @@ def run_uneven(args, torch, H):
     return _dof_line(" (uneven batch)", out[best]["dof_per_s"], "complex DOF/s", args.steps, args.warmup,
                      out[best]["ms_per_launch"],
                      {"workload": f"{P} uneven 2D problems, N log-uniform in {{63..2047}}, k~U[1,80], order~U{{4,6}}, "
-                                  f"manufactured sines (seed {args.seed}); one HIP graph launch per step",
+                                  f"manufactured sines (seed {args.seed}); one HIP graph launch per step, "
+                                  f"{'one plan per (N, order)' if args.by_order else 'one multi-operator plan per N'}",
                       "problems": P, "total_dof": total_dof, "groups": sizes,
                       "tridiagonal_shapes": "built-in" if not args.tune else
                       {f"N{gi['N']}_order{gi['order']}": gi["shapes"] for gi in info}},
