@@ -1021,9 +1021,10 @@ extern "C" int hh_plan_set_tri_shape(hh_plan *p, int W, int rows_per_thread, int
     if (P > n / 2) P = n / 2;
     if (P < 1 || W * P > 1024 || (n + P - 1) / P > rows_per_thread)
         return fail(HH_ERR_UNSUPPORTED, "tridiagonal shape W=%d rows=%d does not fit n=%d", W, rows_per_thread, n);
-    // the reduced system's two PCR buffers (A, C and the right-hand sides; FP64 complex) must fit the
-    // 160 KiB of LDS
-    if ((size_t)2 * (2 + rhs_per_thread) * 2 * P * W * 16 > (size_t)160 * 1024)
+    // the reduced system's two PCR buffers (A, C and the FP64 right-hand sides; A and C real for the
+    // real-band kernel of real operators, complex otherwise) must fit the 160 KiB of LDS.  A shape that
+    // fits only real bands is replaced by the built-in one while the operators are complex.
+    if (hh::tri_lds_bytes(hh_plan_is_real_operator(p) ? 8 : 16, 16, P, W, rhs_per_thread) > (size_t)160 * 1024)
         return fail(HH_ERR_UNSUPPORTED, "tridiagonal shape W=%d rows=%d rhs=%d needs more LDS than a CU has", W,
                     rows_per_thread, rhs_per_thread);
     p->tri_W = W;
@@ -1141,12 +1142,18 @@ extern "C" int hh_plan_get_tri_shape(const hh_plan *p, int *W, int *P, int *rows
     // the shape of the solve's elimination (mixed precision: FP32), for a
     // batch of two or more right-hand sides per operator
     builtin_tri_shape(p, p->prec == HH_PREC_MIXED ? 2 : 0, 2, p->real_op && p->real_mode != 0, t);
-    if (p->tri_W > 0) {
-        t.W = p->tri_W;
-        t.mmax = p->tri_mm;
-        t.nr = p->prec == HH_PREC_MIXED ? 1 : p->tri_nr;
-        t.P = (n + t.mmax - 1) / t.mmax;
-        if (t.P > n / 2) t.P = n / 2;
+    if (p->tri_W > 0) {   // the tuned shape, where its reduced system fits the LDS (as in solve_core)
+        const bool mixed = p->prec == HH_PREC_MIXED, real = p->real_op && p->real_mode != 0;
+        const int nr = mixed ? 1 : p->tri_nr;
+        int P2 = (n + p->tri_mm - 1) / p->tri_mm;
+        if (P2 > n / 2) P2 = n / 2;
+        if (hh::tri_lds_bytes((real ? 8 : 16) / (mixed ? 2 : 1), mixed ? 8 : 16, P2, p->tri_W, nr) <=
+            (size_t)160 * 1024) {
+            t.W = p->tri_W;
+            t.mmax = p->tri_mm;
+            t.nr = nr;
+            t.P = P2;
+        }
     }
     if (W) *W = t.W;
     if (P) *P = t.P;
@@ -1217,11 +1224,16 @@ static int solve_core(hh_plan *p, int ti, int tc, int to, int tri_comp, const vo
     const long per_op = (batch + p->nop - 1) / p->nop;
     builtin_tri_shape(p, tri_comp == 1 ? 2 : tc == 1 ? 1 : 0, per_op, t.real, t);
     if (p->tri_W > 0 && p->tri_mm > 0) {   // shape chosen by the autotuner
-        t.W = p->tri_W;
-        t.mmax = p->tri_mm;
-        t.nr = tri_comp == 0 ? p->tri_nr : 1;
-        t.P = (n + t.mmax - 1) / t.mmax;
-        if (t.P > n / 2) t.P = n / 2;
+        const int nr = tri_comp == 0 ? p->tri_nr : 1;
+        int P = (n + p->tri_mm - 1) / p->tri_mm;
+        if (P > n / 2) P = n / 2;
+        const size_t op = (t.real ? 8 : 16) / (tri_comp == 0 ? 1 : 2), cx = tri_comp == 0 ? 16 : 8;
+        if (hh::tri_lds_bytes(op, cx, P, p->tri_W, nr) <= (size_t)160 * 1024) {
+            t.W = p->tri_W;
+            t.mmax = p->tri_mm;
+            t.nr = nr;
+            t.P = P;
+        }
     }
     // near-resonant modes: reference-order substitution from the host's
     // factors (FP64 arithmetic), then the partitioned kernel for the rest

@@ -119,6 +119,38 @@ def test_mixed_3d_255_row_shapes(H, ctx):
         assert err <= 1e-10
 
 
+def test_real_band_shape_and_complex_fallback(H, ctx, oracle):
+    """A shape whose reduced system fits the 160 KiB of LDS only with real
+    bands (W = 8 modes x 128 chunks of 8 rows at N = 1023: 1024 threads,
+    128 KiB) is accepted for a real operator and solves to the oracle; once
+    hh_plan_set_operators makes the operator complex the plan reports and
+    runs its built-in shape instead."""
+    import ctypes
+    N = 1023
+    sr = oracle.solve2d(6, N, 1, [50.0, 0.0, 7, 11, 1.3], threads=8)
+    sc = oracle.solve2d(6, N, 1, [50.0, 0.5, 7, 11, 1.3], threads=8)
+    plan = H.Plan2D(ctx, N, N, sr["coef"])
+
+    def shape():
+        v = [ctypes.c_int() for _ in range(4)]
+        H._check(H.lib().hh_plan_get_tri_shape(plan.h, *[ctypes.byref(x) for x in v]))
+        return [x.value for x in v]
+
+    H._check(H.lib().hh_plan_set_tri_shape(plan.h, 8, 8, 1))
+    assert shape() == [8, 128, 8, 1]
+    e = relerr(plan.solve(dev(sr["rhs"])).cpu().numpy().ravel(), sr["sol"])
+    say(f"real operator, W=8 x 8 rows: err {e:.2e}")
+    assert e <= 1e-12
+    plan.set_operators([sc["coef"]])
+    assert H.lib().hh_plan_is_real_operator(plan.h) == 0
+    assert shape()[0] != 8
+    e = relerr(plan.solve(dev(sc["rhs"])).cpu().numpy().ravel(), sc["sol"])
+    say(f"complex operator, built-in shape {shape()}: err {e:.2e}")
+    assert e <= 1e-12
+    # a complex plan refuses the shape outright
+    assert H.lib().hh_plan_set_tri_shape(plan.h, 8, 8, 1) == H.HH_ERR_UNSUPPORTED
+
+
 def test_set_operators_matches_new_plan(H, ctx, oracle):
     N = 127
     probs = [oracle.solve2d(o, N, 1, [k, 0.0, 3, 2, 0.3], threads=8) for o, k in
