@@ -66,6 +66,25 @@ def test_white_noise_2d(H, ctx, N, k, order):
     assert err <= 1e-12
 
 
+@pytest.mark.parametrize("order,k", [(4, 50.0), (6, 120.0)])
+def test_largest_fft_grid_2d(H, ctx, order, k):
+    """N = 4095: the longest half-length transform (N+1 = 4096) and rows of
+    4095 in the mode solves (256 chunks per mode; the near-resonant chains
+    spread over eight waves).  A manufactured sine and a white-noise right-
+    hand side against the oracle."""
+    N = 4095
+    say(f"largest FFT grid 2D order {order} N={N} k={k}")
+    s = O.solve2d(order, N, O.SINE, [k, 0.0, 37, 11, 0.3], threads=16)
+    plan = H.Plan2D(ctx, N, N, s["coef"])
+    e1 = relerr(plan.solve(dev(s["rhs"])).cpu().numpy(), s["sol"])
+    rng = np.random.default_rng(order)
+    rhs = noise(rng, N * N)
+    ref = O.solver2d(N, N, rhs, s["LU"], threads=16)
+    e2 = relerr(plan.solve(dev(rhs)).cpu().numpy(), ref)
+    say(f"  near-resonant modes {plan.mode_info()[0]}, rel. error vs oracle: sine {e1:.3e}, white noise {e2:.3e}")
+    assert e1 <= 1e-12 and e2 <= 1e-12
+
+
 @pytest.mark.parametrize("N,k", [(127, 14.1), (255, 30.0)])
 @pytest.mark.parametrize("order", [4, 6])
 def test_white_noise_3d(H, ctx, N, k, order):
