@@ -178,3 +178,51 @@ def test_c5_poisson_stream(H):
         checked += 1
     srv.close()
     assert checked == 5
+
+
+def test_3d_511_residual_and_order(H, ctx):
+    """3D order 6, k = 30, u = (1+i) sin(4 pi x) sin(13 pi y) sin(13 pi z)
+    at 255^3 and 511^3 (2.1 GB per field, past what the oracle runs in a
+    test), right-hand sides assembled on the GPU.  Properties that hold at
+    any size: the FP64 solution's residual is at the roundoff of the
+    stencil's terms, |A u - F| <= 1e-12 |u| (F itself is O(h^2 k^2 |u|),
+    1.4e-2 |u| at 511^3, so a wrong mode, sign or band fails by orders of
+    magnitude); the discretisation
+    error against the manufactured solution falls like h^6 (at least 40x
+    from h = 1/256 to 1/512, where a fourth-order slip gives 16x); and the
+    mixed-precision solve (two fixed sweeps) agrees with FP64 to 1e-10."""
+    import math
+    import torch
+    kk, order, abc = 30.0, 6, (4, 13, 13)
+    errs = {}
+    for N in (255, 511):
+        h = 1.0 / (N + 1)
+        k2e = np.full(N + 2, kk * kk, np.complex128)
+        coef = H.coefficients3d(order, N, N, N, h, h, h, k2e)
+        plan = H.Plan3D(ctx, N, N, N, coef)
+        x = torch.arange(N + 2, dtype=torch.float64, device="cuda") * h
+        sx, sy, sz = (torch.sin(m * math.pi * x) for m in abc)
+        u_e = (sz[:, None, None] * sy[None, :, None] * sx[None, None, :]).to(torch.complex128) * (1 + 1j)
+        f_e = (h * h * (kk * kk - math.pi ** 2 * sum(m * m for m in abc))) * u_e
+        zf = torch.zeros(2 * (N + 2) * (N + 2), dtype=torch.complex128, device="cuda")
+        F = plan.rhs(order, h, h, h, f_e, u_e, k2e, fxx_f=zf, fyy_f=zf, fzz_f=zf)
+        u_exact = u_e[1:-1, 1:-1, 1:-1].contiguous()
+        del f_e, u_e
+        u = plan.solve(F)
+        fmax, _ = ctx.norms(F)
+        rmax, _ = ctx.norms(plan.residual(u, F))
+        umax, d = ctx.norms(u, u_exact)
+        errs[N] = d / umax
+        say(f"3D N={N}: residual {rmax / umax:.2e} |u| ({rmax / fmax:.2e} |F|), discretisation error {errs[N]:.3e}")
+        assert rmax <= 1e-12 * umax
+        if N == 511:
+            pm = H.Plan3D(ctx, N, N, N, coef, precision=H.PREC_MIXED)
+            pm.set_refine(0.0, 2)
+            _, dm = ctx.norms(pm.solve(F), u)
+            say(f"  mixed precision vs FP64: {dm / umax:.2e}")
+            assert dm / umax <= 1e-10
+            del pm
+        del plan, F, u, u_exact
+        torch.cuda.empty_cache()
+    say(f"  error ratio 255 -> 511: {errs[255] / errs[511]:.1f}")
+    assert errs[255] / errs[511] >= 40
