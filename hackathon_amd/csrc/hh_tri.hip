@@ -277,7 +277,12 @@ __global__ __launch_bounds__(NT) void k_tri(Prob prob, long nmodes, int W, int P
     const int w = threadIdx.x % W, c = threadIdx.x / W;
     const long m = xcd_tile(blockIdx.x, gridDim.x) * W + w;
     const bool valid = (c < P) && (m < nmodes);
-    const int o = blockIdx.y % nop, jg = g0 + (int)(blockIdx.y / nop);
+    // right-hand-side groups in reverse launch order: the row transform
+    // before this pass wrote the last right-hand sides last, so they are the
+    // ones still in the memory-side cache when the pass starts (and the
+    // first ones this pass writes last, where the next row transform starts)
+    const int ng = (int)(gridDim.y / nop);
+    const int o = blockIdx.y % nop, jg = g0 + (ng - 1 - (int)(blockIdx.y / nop));
     int bidx[NR];
     bool bok[NR];
 #pragma unroll
@@ -714,10 +719,11 @@ static hipError_t launch_tri_nr(const Prob &pb, int n, long nmodes, int batch, i
     }
     const int per_op = (batch + nop - 1) / nop;
     const int groups = (per_op + NR - 1) / NR;
-    // grid y holds at most 65535 blocks: launch the right-hand-side groups in slices
+    // grid y holds at most 65535 blocks: launch the right-hand-side groups in
+    // slices, the last slice first (the kernel walks its groups backwards too)
     if (nop > 65535) return hipErrorInvalidValue;
     const int gmax = 65535 / nop;
-    for (int g0 = 0; g0 < groups; g0 += gmax) {
+    for (int g0 = ((groups - 1) / gmax) * gmax; g0 >= 0; g0 -= gmax) {
         const int ng = groups - g0 < gmax ? groups - g0 : gmax;
         dim3 grid((unsigned)((nmodes + W - 1) / W), (unsigned)(nop * ng));
         hipLaunchKernelGGL(kfn, grid, dim3(threads), shm, st, pb, nmodes, W, P, batch, nop, g0);
